@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark of the k-space SBE time loop on one MI355X.
+
+Workload (default): the plain FP64 path at README scale -- 2-band
+Haldane-like honeycomb (tests/golden/haldane_tb.dat, regenerated by
+hhgamd.tbgen), 200x200 k-grid, nr3 = 1, complex128 rho, dt = 0.02197,
+tmax = 90 (4096 RK4 steps), lambda = 3000 nm, I = 1e12 W/cm^2, sin^2 envelope
+to 80, polarisation along y, CEP 0.
+
+A "step" is one full solve over the resident inputs: 4096 RK4 steps of all
+40 000 k-points plus the reduction of J(t) and its copy to the host
+(SWESim_run_simulation). Inputs (blocks, field tables, initial rho) are in
+HBM before the timed region. value = k-points * RK4 steps * K / wall time.
+
+Printed JSON also carries: latency percentiles of the solves, J(t) relative
+L2 error against the FP64 CPU oracle at full size, the roofline of the
+propagation kernel (HIP events on its stream) and the CPU baseline (the
+oracle's faithful real-space FFT restatement of the reference algorithm,
+single-threaded like the reference, on a bounded sample of steps).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "hhg-solids_amd")]
+
+METRIC = "k-point·RK4-steps/s; p50/p99 solve latency; J(t) rel-L2 err vs FP64"
+UNIT = "k-point·RK4-steps/s"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_HBM_GBS = 8000.0
+
+
+def workload_param(path_tb, nr, precision):
+    return {"path_tb": path_tb, "nr1": nr, "nr2": nr, "nr3": 1, "tmax": 90.0, "dt": 0.02197,
+            "intensity": 1e12, "lambda": 3000.0, "tmax_field": 80.0,
+            "pol_vec": np.array([0.0, 1.0, 0.0]), "phi_vec": np.array([0.0, 0.0, 0.0]),
+            "precision": precision}
+
+
+def algorithmic_flops_per_kstep(nb, nR):
+    """8 (8 nb^3 + 4 nR nb^2 + 48 nb^2): 4 stages x 2 complex nb x nb products,
+    the H and three rbc Fourier sums over nR blocks, the windowed J of three
+    axes over 16 points (complex multiply-add = 8 flops)."""
+    return 8 * (8 * nb ** 3 + 4 * nR * nb ** 2 + 48 * nb ** 2)
+
+
+def nonzero_blocks(H, r):
+    return int(sum(1 for i in range(len(H)) if np.any(H[i] != 0) or np.any(r[:, i] != 0)))
+
+
+def cpu_baseline(path_tb, nr, sample_steps):
+    import oracle
+    prm = workload_param(path_tb, nr, "fp64")
+    prm["tmax"] = 0.02197 * sample_steps + 1e-9
+    n = oracle.nt(prm)
+    t0 = time.perf_counter()
+    oracle.run(path_tb, prm, mode="fft")
+    dt = time.perf_counter() - t0
+    return {"value": nr * nr * n / dt, "unit": UNIT, "cores": 1, "kind": "port",
+            "sample": f"oracle FFT-form restatement of the reference real-space loop (project-written FFT, "
+                      f"not FFTW), {nr}x{nr} k-grid, {n} RK4 steps of the same workload, setup included, "
+                      f"{dt:.1f} s"}
+
+
+def cpu_perk_threads(path_tb, nr, steps, threads):
+    import oracle
+    prm = workload_param(path_tb, nr, "fp64")
+    prm["tmax"] = 0.02197 * steps + 1e-9
+    n = oracle.nt(prm)
+    t0 = time.perf_counter()
+    oracle.run(path_tb, prm, mode="perk", nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": nr * nr * n / dt, "cores": threads, "steps": n, "seconds": round(dt, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nr", type=int, default=200)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the full-size oracle J check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
+              "(the k-grid would shard by k-slabs with one final reduce of J, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from hhgamd import lib, tbgen
+
+    tmp = tempfile.mkdtemp(prefix="hhg_bench_")
+    path_tb = os.path.join(tmp, "haldane_tb.dat")
+    model = tbgen.haldane()
+    tbgen.write_tb(path_tb, *model)
+    prm = workload_param(path_tb, args.nr, args.precision)
+
+    t_setup = time.perf_counter()
+    s = lib.Solve(prm)
+    setup_s = time.perf_counter() - t_setup
+    nk, nt, nb = s.N, s.nt, s.nb
+    for _ in range(args.warmup):
+        s.run()
+    lat, kern = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        s.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+        kern.append(s.kernel_ms())
+    wall = time.perf_counter() - t0
+    _, J = s.current()  # first solve after init (the warmup's first run)
+    s.close()
+
+    value = nk * nt * args.steps / wall
+    kms = float(np.mean(kern))
+    nR = nonzero_blocks(model[2], model[3])
+    F = algorithmic_flops_per_kstep(nb, nR)
+    achieved = F * nk * nt / (kms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"haldane-2band-{args.nr}x{args.nr}-nt{nt}-{args.precision}", "nk": nk, "nb": nb,
+                   "nt": nt, "nR_nonzero": nR, "precision": args.precision, "setup_s": round(setup_s, 3)},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "roofline": {"bound": "mfma", "alu": "FP64 VALU (v_fma_f64): MI355X FP64 vector peak = FP64 matrix peak",
+                     "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
+                     "kernel_ms": kms, "flops_per_kstep": F,
+                     "hbm_view_GBs": 2 * 16 * nb * nb * nk * nt / (kms * 1e-3) / 1e9},
+    }
+    prof = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        tr = json.load(open(prof)).get(out["config"]["workload"])
+        if tr:
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
+    if not args.no_cpu:  # CPU leg: the oracle, as baseline and as the FP64 checker of J(t)
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(path_tb, args.nr, args.cpu_steps)
+        out["cpu_perk_threads"] = cpu_perk_threads(path_tb, args.nr, 64, threads)
+        if not args.no_check:
+            import oracle
+            _, Jref = oracle.run(path_tb, prm, mode="perk", nthreads=threads)
+            out["j_rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(J - Jref) / np.linalg.norm(Jref))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

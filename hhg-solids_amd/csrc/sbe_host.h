@@ -1,0 +1,58 @@
+// sbe_host.h -- host-side model, setup tables and the device interface.
+#pragma once
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "sbe_types.h"
+
+namespace hhg {
+
+using cplx = std::complex<double>;
+
+// Wannier tight-binding model as read from a W90 _tb.dat file
+// (src/wannier_tb.cpp:15-113). Blocks are stored the way the reference
+// stores them: file line "m n re im" at [n-1][m-1].
+struct TBModel {
+    std::string comment;
+    double lat[3][3];
+    int nb = 0, nR = 0;
+    std::vector<int> deg;
+    std::vector<int> R;              // nR*3
+    std::vector<cplx> H;             // nR*nb*nb, eV
+    std::vector<cplx> r[3];          // nR*nb*nb, Angstrom
+};
+TBModel read_tb(const std::string& path);
+
+// Parameters of one solve; same meaning and units as include/settings.h:5-31.
+struct SolveParams {
+    int nr1 = 400, nr2 = 400, nr3 = 1;
+    double tmax = 90.0, dt = 10.98e-3 / 0.5;
+    double lambda_nm = 3000, tmax_field = 80.0, intensity = 5e10;
+    double s[3] = {0.96, 0.25, 0.0}, phi[3] = {0.0, 0.0, 0.0};
+};
+
+// Everything the device needs for one solve, built on the host in FP64.
+struct HostSolve {
+    SolveDesc d;
+    double dt = 0;                   // t[1]-t[0] (au)
+    std::vector<double> blk;         // ns*NOPS*nb*nb complex
+    std::vector<double> steptab;     // nt*stride
+    std::vector<double> tw1, tw2;    // N1, N2 complex
+    std::vector<double> rho0;        // nb*nb*N complex, element-major
+    std::vector<double> t, E[3], A[3];
+};
+HostSolve build_solve(const TBModel& m, const SolveParams& p);
+
+// Device side (sbe_kernels.hip).
+struct DeviceSolve;
+DeviceSolve* dev_create(const HostSolve& hs);
+// Propagates nt steps from the current device state. J_out: [nt][6].
+// Returns the device time in milliseconds.
+double dev_run(DeviceSolve* ds, int precision, double* J_out);
+void dev_get_rho(DeviceSolve* ds, double* rho_elem_major);
+// Device time of the propagation kernel alone in the last dev_run.
+double dev_last_kernel_ms(DeviceSolve* ds);
+void dev_destroy(DeviceSolve* ds);
+
+}  // namespace hhg

@@ -1,0 +1,588 @@
+// sbe_kernels.hip -- CDNA4 (gfx950) kernels of the k-space SBE time loop.
+//
+// Every k-point is an independent nb x nb RK4 system (the per-k form of
+// src/solver.cpp:86-181,1428-1726), so the whole time loop runs inside one
+// launch: each k keeps its density matrix in registers for all nt steps and
+// only the per-step current partials leave the CU. Per step and k:
+//
+//   w_s   = e^{-2 pi i (m1 n1_s/N1 + m2 n2_s/N2)} * e^{i A(t_i).R_s}
+//   H~    = sum_s w_s H_s,   X~_a = sum_s w_s rbc_a,s      (Wannier Fourier sums)
+//   K_j   = -i [H~ + E_j . X~, rho_j]                     (4 RK4 stages)
+//   rho  += 0.1666666666666 dt (K1 + 2 K2 + 2 K3 + K4)
+//   J_a  += sum_s w_s sum_ij v_a,s[ij] conj(rho_ij)       (4x4 window, src/observable.cpp:16-30)
+//
+// Kernels:
+//   k_thread_per_k<NB,MAXU,T>  one lane per k, everything in registers; the
+//                              blocks and the per-step table are uniform and
+//                              arrive through scalar loads. For nb <= 2.
+//   k_elem_per_lane<EPL,T>     G = pow2(nb*nb) <= 64 lanes per k, each lane
+//                              owns EPL matrix elements; the stage
+//                              Hamiltonian and stage rho are staged through
+//                              LDS for the commutator. Any nb <= 16.
+//   k_reduce_j                 deterministic sum of the per-wave partials.
+// T is the stage arithmetic (double for the plain path, float for the mixed
+// path); the state rho and the current are always FP64.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "sbe_host.h"
+
+namespace hhg {
+
+#define HIP_CHECK(x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                     " at " __FILE__ ":" + std::to_string(__LINE__));       \
+    } while (0)
+
+template <typename T>
+struct cx {
+    T re, im;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ void cmac(cx<T>& acc, cx<T> a, cx<T> b) {  // acc += a*b
+    acc.re += a.re * b.re - a.im * b.im;
+    acc.im += a.re * b.im + a.im * b.re;
+}
+template <typename T>
+__device__ __forceinline__ void cmsub(cx<T>& acc, cx<T> a, cx<T> b) {  // acc -= a*b
+    acc.re -= a.re * b.re - a.im * b.im;
+    acc.im -= a.re * b.im + a.im * b.re;
+}
+template <typename T, typename S>
+__device__ __forceinline__ cx<T> cast(cx<S> a) {
+    return {(T)a.re, (T)a.im};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld(const double* p) {  // complex from interleaved doubles
+    return {(T)p[0], (T)p[1]};
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int posmod(long a, int n) {
+    int r = (int)(a % n);
+    return r < 0 ? r + n : r;
+}
+
+// --------------------------------------------------------------------------
+// One lane per k. MAXU bounds the support size so that the per-k DFT phases
+// live in registers.
+template <int NB, int MAXU, typename T>
+__global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const double* __restrict__ blk,
+                                                     const double* __restrict__ steptab,
+                                                     const double* __restrict__ tw1,
+                                                     const double* __restrict__ tw2,
+                                                     const double* __restrict__ rho_in,
+                                                     double* __restrict__ rho_out, double* __restrict__ jpart,
+                                                     int nwaves, double dt) {
+    constexpr int E = NB * NB;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = k >> 6;
+    const int lane = threadIdx.x & 63;
+
+    cx<double> w0[MAXU];
+#pragma unroll
+    for (int s = 0; s < MAXU; ++s) {
+        if (s < d.ns) {
+            const int j1 = posmod((long)m1 * d.n1[s], d.N1), j2 = posmod((long)m2 * d.n2[s], d.N2);
+            w0[s] = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
+        } else {
+            w0[s] = {0.0, 0.0};
+        }
+    }
+    cx<double> rho[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        rho[e] = active ? ld<double>(rho_in + 2 * ((size_t)e * d.N + k)) : cx<double>{0.0, 0.0};
+
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    for (int ti = 0; ti < d.nt; ++ti) {
+        const double* st = steptab + (size_t)ti * d.stride;
+        cx<T> Ht[E], Xt[3][E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            Ht[e] = {0, 0};
+            Xt[0][e] = Xt[1][e] = Xt[2][e] = {0, 0};
+        }
+#pragma unroll
+        for (int s = 0; s < MAXU; ++s) {
+            if (s < d.ns) {
+                const uint32_t om = d.opmask[s];
+                const cx<T> w = cast<T>(cmul(w0[s], ld<double>(st + STEP_HEAD + 2 * s)));
+                const double* b = blk + (size_t)s * NOPS * E * 2;
+                if (om & (1u << OP_H)) {
+#pragma unroll
+                    for (int e = 0; e < E; ++e) cmac(Ht[e], w, ld<T>(b + 2 * (OP_H * E + e)));
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (om & (1u << (OP_X + a))) {
+#pragma unroll
+                        for (int e = 0; e < E; ++e) cmac(Xt[a][e], w, ld<T>(b + 2 * ((OP_X + a) * E + e)));
+                    }
+            }
+        }
+        cx<T> Hs[E], rs[E], K[E], acc[E];
+        auto stage_h = [&](int j) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                cx<T> h = Ht[e];
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (d.xmask & (1 << a)) {
+                        const T f = (T)st[3 * j + a];
+                        h.re += f * Xt[a][e].re;
+                        h.im += f * Xt[a][e].im;
+                    }
+                Hs[e] = h;
+            }
+        };
+        auto deriv = [&]() {  // K = -i (Hs rs - rs Hs)
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    cx<T> c = {0, 0};
+#pragma unroll
+                    for (int l = 0; l < NB; ++l) {
+                        cmac(c, Hs[i * NB + l], rs[l * NB + j]);
+                        cmsub(c, rs[i * NB + l], Hs[l * NB + j]);
+                    }
+                    K[i * NB + j] = {c.im, -c.re};
+                }
+        };
+        stage_h(0);
+#pragma unroll
+        for (int e = 0; e < E; ++e) rs[e] = cast<T>(rho[e]);
+        deriv();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            acc[e] = K[e];
+            rs[e] = {(T)rho[e].re + half * K[e].re, (T)rho[e].im + half * K[e].im};
+        }
+        stage_h(1);
+        deriv();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            acc[e].re += (T)2 * K[e].re; acc[e].im += (T)2 * K[e].im;
+            rs[e] = {(T)rho[e].re + half * K[e].re, (T)rho[e].im + half * K[e].im};
+        }
+        deriv();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            acc[e].re += (T)2 * K[e].re; acc[e].im += (T)2 * K[e].im;
+            rs[e] = {(T)rho[e].re + full * K[e].re, (T)rho[e].im + full * K[e].im};
+        }
+        stage_h(2);
+        deriv();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            acc[e].re += K[e].re; acc[e].im += K[e].im;
+            rho[e].re += pref * (double)acc[e].re;
+            rho[e].im += pref * (double)acc[e].im;
+        }
+        // current of the new state with the phases of step ti
+        double jc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < MAXU; ++s) {
+            if (s < d.ns) {
+                const uint32_t om = d.opmask[s];
+                if (!(om & (7u << OP_V))) continue;
+                const cx<double> w = cmul(w0[s], ld<double>(st + STEP_HEAD + 2 * s));
+                const double* b = blk + (size_t)s * NOPS * E * 2;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (om & (1u << (OP_V + a))) {
+                        cx<double> part = {0.0, 0.0};
+#pragma unroll
+                        for (int e = 0; e < E; ++e)
+                            cmac(part, ld<double>(b + 2 * ((OP_V + a) * E + e)), cx<double>{rho[e].re, -rho[e].im});
+                        const cx<double> c = cmul(part, w);
+                        jc[2 * a] += c.re;
+                        jc[2 * a + 1] += c.im;
+                    }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) jc[c] = wave_sum(jc[c]);
+        if (lane == 0) {
+            double* out = jpart + ((size_t)ti * nwaves + wave) * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = jc[c];
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            rho_out[2 * ((size_t)e * d.N + k)] = rho[e].re;
+            rho_out[2 * ((size_t)e * d.N + k) + 1] = rho[e].im;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Element-per-lane kernel for any nb <= 16. A group of G lanes (G = nb*nb
+// rounded up to a power of two, at most 64) owns one k; lane l owns the
+// matrix elements e = l + G*q, q < EPL. Per stage the stage Hamiltonian and
+// stage rho are staged through LDS and each lane forms its commutator
+// elements from row i / column j there. LDS per k: the support phases (ns
+// complex, FP64) and two nb*nb matrices of type T.
+template <int EPL, typename T>
+__global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const double* __restrict__ blk,
+                                                       const double* __restrict__ steptab,
+                                                       const double* __restrict__ tw1,
+                                                       const double* __restrict__ tw2,
+                                                       const double* __restrict__ rho_in,
+                                                       double* __restrict__ rho_out, double* __restrict__ jpart,
+                                                       int nwaves, double dt, int G) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int NB = d.nb, E = NB * NB;
+    const int kpb = blockDim.x / G;
+    const int grp = threadIdx.x / G;
+    const int l = threadIdx.x - grp * G;
+    const int k = blockIdx.x * kpb + grp;
+    const bool kvalid = k < d.N;
+    const int kk = kvalid ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+
+    cx<double>* wS = reinterpret_cast<cx<double>*>(smem) + (size_t)grp * d.ns;
+    cx<T>* lH = reinterpret_cast<cx<T>*>(reinterpret_cast<cx<double>*>(smem) + (size_t)kpb * d.ns) + (size_t)grp * 2 * E;
+    cx<T>* lR = lH + E;
+
+    int ei[EPL], eI[EPL], eJ[EPL];
+    bool ev[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) {
+        const int e = l + G * q;
+        ev[q] = kvalid && e < E;
+        ei[q] = e < E ? e : 0;
+        eI[q] = ei[q] / NB;
+        eJ[q] = ei[q] - eI[q] * NB;
+    }
+    cx<double> rho[EPL];
+#pragma unroll
+    for (int q = 0; q < EPL; ++q)
+        rho[q] = ev[q] ? ld<double>(rho_in + 2 * ((size_t)ei[q] * d.N + k)) : cx<double>{0.0, 0.0};
+
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    for (int ti = 0; ti < d.nt; ++ti) {
+        const double* st = steptab + (size_t)ti * d.stride;
+        for (int s = l; s < d.ns; s += G) {
+            const int j1 = posmod((long)m1 * d.n1[s], d.N1), j2 = posmod((long)m2 * d.n2[s], d.N2);
+            wS[s] = cmul(cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2)), ld<double>(st + STEP_HEAD + 2 * s));
+        }
+        __syncthreads();
+        cx<T> Ht[EPL], Xt[3][EPL];
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+            Ht[q] = {0, 0};
+            Xt[0][q] = Xt[1][q] = Xt[2][q] = {0, 0};
+        }
+        for (int s = 0; s < d.ns; ++s) {
+            const uint32_t om = d.opmask[s];
+            const cx<T> w = cast<T>(wS[s]);
+            const double* b = blk + (size_t)s * NOPS * E * 2;
+            if (om & (1u << OP_H)) {
+#pragma unroll
+                for (int q = 0; q < EPL; ++q) cmac(Ht[q], w, ld<T>(b + 2 * (OP_H * E + ei[q])));
+            }
+            for (int a = 0; a < 3; ++a)
+                if (om & (1u << (OP_X + a))) {
+#pragma unroll
+                    for (int q = 0; q < EPL; ++q) cmac(Xt[a][q], w, ld<T>(b + 2 * ((OP_X + a) * E + ei[q])));
+                }
+        }
+        cx<T> Hs[EPL], rs[EPL], K[EPL], acc[EPL];
+        auto stage_h = [&](int js) {
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) {
+                cx<T> h = Ht[q];
+                for (int a = 0; a < 3; ++a)
+                    if (d.xmask & (1 << a)) {
+                        const T f = (T)st[3 * js + a];
+                        h.re += f * Xt[a][q].re;
+                        h.im += f * Xt[a][q].im;
+                    }
+                Hs[q] = h;
+            }
+        };
+        auto deriv = [&](bool new_h) {  // K = -i (Hs rs - rs Hs), element-wise
+#pragma unroll
+            for (int q = 0; q < EPL; ++q)
+                if (l + G * q < E) {
+                    if (new_h) lH[ei[q]] = Hs[q];
+                    lR[ei[q]] = rs[q];
+                }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) {
+                cx<T> c = {0, 0};
+                const cx<T>* hrow = lH + eI[q] * NB;
+                const cx<T>* rrow = lR + eI[q] * NB;
+#pragma unroll 4
+                for (int m = 0; m < NB; ++m) {
+                    cmac(c, hrow[m], lR[m * NB + eJ[q]]);
+                    cmsub(c, rrow[m], lH[m * NB + eJ[q]]);
+                }
+                K[q] = {c.im, -c.re};
+            }
+            __syncthreads();
+        };
+        stage_h(0);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) rs[q] = cast<T>(rho[q]);
+        deriv(true);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+            acc[q] = K[q];
+            rs[q] = {(T)rho[q].re + half * K[q].re, (T)rho[q].im + half * K[q].im};
+        }
+        stage_h(1);
+        deriv(true);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+            acc[q].re += (T)2 * K[q].re; acc[q].im += (T)2 * K[q].im;
+            rs[q] = {(T)rho[q].re + half * K[q].re, (T)rho[q].im + half * K[q].im};
+        }
+        deriv(false);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+            acc[q].re += (T)2 * K[q].re; acc[q].im += (T)2 * K[q].im;
+            rs[q] = {(T)rho[q].re + full * K[q].re, (T)rho[q].im + full * K[q].im};
+        }
+        stage_h(2);
+        deriv(true);
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+            acc[q].re += K[q].re; acc[q].im += K[q].im;
+            if (ev[q]) {
+                rho[q].re += pref * (double)acc[q].re;
+                rho[q].im += pref * (double)acc[q].im;
+            }
+        }
+        double jc[6] = {0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < d.ns; ++s) {
+            const uint32_t om = d.opmask[s];
+            if (!(om & (7u << OP_V))) continue;
+            const cx<double> w = wS[s];
+            const double* b = blk + (size_t)s * NOPS * E * 2;
+            for (int a = 0; a < 3; ++a)
+                if (om & (1u << (OP_V + a))) {
+                    cx<double> part = {0.0, 0.0};
+#pragma unroll
+                    for (int q = 0; q < EPL; ++q)
+                        if (ev[q]) cmac(part, ld<double>(b + 2 * ((OP_V + a) * E + ei[q])), cx<double>{rho[q].re, -rho[q].im});
+                    const cx<double> c = cmul(part, w);
+                    jc[2 * a] += c.re;
+                    jc[2 * a + 1] += c.im;
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) jc[c] = wave_sum(jc[c]);
+        if (lane == 0) {
+            double* out = jpart + ((size_t)ti * nwaves + wave) * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = jc[c];
+        }
+        __syncthreads();  // wS is rewritten by the next step
+    }
+#pragma unroll
+    for (int q = 0; q < EPL; ++q)
+        if (ev[q]) {
+            rho_out[2 * ((size_t)ei[q] * d.N + k)] = rho[q].re;
+            rho_out[2 * ((size_t)ei[q] * d.N + k) + 1] = rho[q].im;
+        }
+}
+
+// J[ti][c] = (sum over waves, in wave order per lane stripe) / N
+__global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, double N,
+                                                 double* __restrict__ J) {
+    const int ti = blockIdx.x;
+    const int lane = threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = lane; w < nwaves; w += 64) {
+        const double* p = jpart + ((size_t)ti * nwaves + w) * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += p[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) J[(size_t)ti * 6 + c] = acc[c] / N;
+    }
+}
+
+// --------------------------------------------------------------------------
+struct DeviceSolve {
+    SolveDesc d;
+    double dt = 0;
+    double *blk = nullptr, *steptab = nullptr, *tw1 = nullptr, *tw2 = nullptr;
+    double *rho = nullptr, *rho_tmp = nullptr, *jpart = nullptr, *J = nullptr;
+    size_t jpart_bytes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
+    float last_kernel_ms = 0.f;
+};
+
+template <typename V>
+static double* upload(const std::vector<V>& h, hipStream_t s) {
+    double* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, sizeof(V) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HIP_CHECK(hipMemcpyAsync(p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+    return p;
+}
+
+DeviceSolve* dev_create(const HostSolve& hs) {
+    DeviceSolve* ds = new DeviceSolve();
+    try {
+        ds->d = hs.d;
+        ds->dt = hs.dt;
+        HIP_CHECK(hipStreamCreateWithFlags(&ds->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&ds->ev0));
+        HIP_CHECK(hipEventCreate(&ds->ev1));
+        HIP_CHECK(hipEventCreate(&ds->evk));
+        ds->blk = upload(hs.blk, ds->stream);
+        ds->steptab = upload(hs.steptab, ds->stream);
+        ds->tw1 = upload(hs.tw1, ds->stream);
+        ds->tw2 = upload(hs.tw2, ds->stream);
+        ds->rho = upload(hs.rho0, ds->stream);
+        HIP_CHECK(hipMalloc(&ds->rho_tmp, sizeof(double) * hs.rho0.size()));
+        HIP_CHECK(hipMalloc(&ds->J, sizeof(double) * 6 * (size_t)hs.d.nt));
+        HIP_CHECK(hipStreamSynchronize(ds->stream));
+    } catch (...) {
+        dev_destroy(ds);
+        throw;
+    }
+    return ds;
+}
+
+void dev_destroy(DeviceSolve* ds) {
+    if (!ds) return;
+    for (double* p : {ds->blk, ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, ds->J})
+        if (p) (void)hipFree(p);
+    if (ds->ev0) (void)hipEventDestroy(ds->ev0);
+    if (ds->ev1) (void)hipEventDestroy(ds->ev1);
+    if (ds->evk) (void)hipEventDestroy(ds->evk);
+    if (ds->stream) (void)hipStreamDestroy(ds->stream);
+    delete ds;
+}
+
+static int pow2ceil(int n) {
+    int g = 1;
+    while (g < n) g <<= 1;
+    return g;
+}
+
+static void ensure_jpart(DeviceSolve* ds, int nwaves) {
+    size_t need = sizeof(double) * 6 * (size_t)ds->d.nt * nwaves;
+    if (need > ds->jpart_bytes) {
+        if (ds->jpart) HIP_CHECK(hipFree(ds->jpart));
+        ds->jpart = nullptr;
+        HIP_CHECK(hipMalloc(&ds->jpart, need));
+        ds->jpart_bytes = need;
+    }
+}
+
+template <int EPL, typename T>
+static void launch_elems(DeviceSolve* ds, int& nwaves) {
+    const SolveDesc& d = ds->d;
+    const int E = d.nb * d.nb;
+    const int G = std::min(64, pow2ceil(E));
+    if ((E + G - 1) / G > EPL) throw std::runtime_error("k_elem_per_lane: too many elements per lane");
+    const int block = 256;
+    const int kpb = block / G;
+    const int nblk = (d.N + kpb - 1) / kpb;
+    nwaves = nblk * (block / 64);
+    ensure_jpart(ds, nwaves);
+    size_t lds = sizeof(cx<double>) * (size_t)kpb * d.ns + sizeof(cx<T>) * (size_t)kpb * 2 * E;
+    if (lds > 160 * 1024) throw std::runtime_error("k_elem_per_lane: LDS footprint exceeds 160 KiB");
+    if (lds > 64 * 1024)
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_elem_per_lane<EPL, T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_elem_per_lane<EPL, T>), dim3(nblk), dim3(block), lds, ds->stream, d, ds->blk, ds->steptab,
+                       ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt, G);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int NB, int MAXU, typename T>
+static void launch_threads(DeviceSolve* ds, int& nwaves) {
+    const SolveDesc& d = ds->d;
+    const int block = 64;
+    const int nblk = (d.N + block - 1) / block;
+    nwaves = nblk;
+    ensure_jpart(ds, nwaves);
+    hipLaunchKernelGGL((k_thread_per_k<NB, MAXU, T>), dim3(nblk), dim3(block), 0, ds->stream, d, ds->blk,
+                       ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+static void launch_any(DeviceSolve* ds, int& nwaves) {
+    const SolveDesc& d = ds->d;
+    if (d.nb == 1 && d.ns <= 16) return launch_threads<1, 16, T>(ds, nwaves);
+    if (d.nb == 2 && d.ns <= 8) return launch_threads<2, 8, T>(ds, nwaves);
+    if (d.nb == 2 && d.ns <= 16) return launch_threads<2, 16, T>(ds, nwaves);
+    if (d.nb == 2 && d.ns <= 32) return launch_threads<2, 32, T>(ds, nwaves);
+    if (d.nb * d.nb <= 64) return launch_elems<1, T>(ds, nwaves);
+    if (d.nb <= 16) return launch_elems<4, T>(ds, nwaves);
+    throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
+}
+
+double dev_run(DeviceSolve* ds, int precision, double* J_out) {
+    const SolveDesc& d = ds->d;
+    int nwaves = 0;
+    HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
+    if (precision == 0)
+        launch_any<double>(ds, nwaves);
+    else
+        launch_any<float>(ds, nwaves);
+    HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
+    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, (double)d.N, ds->J);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(ds->ev1, ds->stream));
+    std::swap(ds->rho, ds->rho_tmp);
+    HIP_CHECK(hipMemcpyAsync(J_out, ds->J, sizeof(double) * 6 * (size_t)d.nt, hipMemcpyDeviceToHost, ds->stream));
+    HIP_CHECK(hipStreamSynchronize(ds->stream));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ds->ev0, ds->ev1));
+    HIP_CHECK(hipEventElapsedTime(&ds->last_kernel_ms, ds->ev0, ds->evk));
+    return ms;
+}
+
+double dev_last_kernel_ms(DeviceSolve* ds) { return ds ? ds->last_kernel_ms : 0.0; }
+
+void dev_get_rho(DeviceSolve* ds, double* out) {
+    const SolveDesc& d = ds->d;
+    HIP_CHECK(hipMemcpyAsync(out, ds->rho, sizeof(double) * 2 * (size_t)d.nb * d.nb * d.N, hipMemcpyDeviceToHost,
+                             ds->stream));
+    HIP_CHECK(hipStreamSynchronize(ds->stream));
+}
+
+}  // namespace hhg

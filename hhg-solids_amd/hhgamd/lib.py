@@ -1,0 +1,137 @@
+"""Loading libwannier.so and the list of symbols its C ABI exports."""
+import ctypes as ct
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from . import LIB_PATH, PKG_DIR, REPO_DIR
+
+HEADER = os.path.join(REPO_DIR, "include", "hhg_swe.h")
+
+
+def build(jobs=8):
+    """Compile libwannier.so in-tree for gfx950 (hipcc cross-compiles without a GPU)."""
+    subprocess.run(["make", "-s", "-C", PKG_DIR, f"-j{jobs}"], check=True)
+
+
+def load(path=LIB_PATH):
+    """Load the native library; fails loudly when it has not been built."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `make -C hhg-solids_amd`")
+    return ct.CDLL(path)
+
+
+def declared_symbols(header=HEADER):
+    """Function names declared in include/hhg_swe.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)) - {"if", "defined"})
+
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C")
+_C = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C")
+
+
+def bind(lib):
+    """Declare argtypes/restypes of every C-ABI function of include/hhg_swe.h."""
+    V, I, Dd, P = ct.c_void_p, ct.c_int, ct.c_double, ct.c_char_p
+    sig = {
+        "SWESim_new": ([], V), "SWESim_run_simulation": ([V], None), "SWESim_test_files": ([V], None),
+        "SWESim_set_path_tb": ([V, P], None), "SWESim_set_settings": ([V, V], None),
+        "SWESim_init": ([V], None), "SWESim_restart": ([V], None), "SWESim_init_host": ([V], None),
+        "SWESim_get_current": ([V, _D, _C, _C, _C], None), "SWESim_delete": ([V], None),
+        "Settings_new": ([], V), "Settings_delete": ([V], None),
+        "Settings_set_nr1": ([V, I], None), "Settings_set_nr2": ([V, I], None), "Settings_set_nr3": ([V, I], None),
+        "Settings_set_tmax": ([V, Dd], None), "Settings_set_dt": ([V, Dd], None),
+        "Settings_set_intensity": ([V, Dd], None), "Settings_set_lambda": ([V, Dd], None),
+        "Settings_set_tmax_field": ([V, Dd], None), "Settings_set_pol_vec": ([V, _D], None),
+        "Settings_set_phi_vec": ([V, _D], None),
+        "SWESim_get_status": ([V], I), "SWESim_get_error": ([V], P), "hhg_last_error": ([], P),
+        "SWESim_get_nt": ([V], I), "SWESim_get_num_orb": ([V], I), "SWESim_set_precision": ([V, I], None),
+        "SWESim_get_fields": ([V, _D, _D, _D], I), "SWESim_get_rho_k": ([V, _C], I),
+        "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+class Solve:
+    """One solve behind the C ABI, with the extension calls.
+
+    ``param`` uses the keys of the reference's pyswe dictionary (nr1, nr2,
+    nr3, tmax, dt, intensity, lambda, tmax_field, pol_vec, phi_vec) plus
+    path_tb and optionally precision ("fp64" | "fp32").
+    """
+
+    def __init__(self, param, device=True, lib=None):
+        self.lib = bind(lib or load())
+        L = self.lib
+        self.st = L.Settings_new()
+        for k in ("nr1", "nr2", "nr3"):
+            getattr(L, "Settings_set_" + k)(self.st, int(param.get(k, 1)))
+        for k, f in (("tmax", "tmax"), ("dt", "dt"), ("intensity", "intensity"), ("lambda", "lambda"),
+                     ("tmax_field", "tmax_field")):
+            getattr(L, "Settings_set_" + f)(self.st, float(param[k]))
+        L.Settings_set_pol_vec(self.st, np.ascontiguousarray(param["pol_vec"], dtype=np.float64))
+        L.Settings_set_phi_vec(self.st, np.ascontiguousarray(param["phi_vec"], dtype=np.float64))
+        self.ptr = L.SWESim_new()
+        L.SWESim_set_path_tb(self.ptr, param["path_tb"].encode())
+        L.SWESim_set_settings(self.ptr, self.st)
+        prec = {"fp64": 0, "fp32": 1}.get(param.get("precision", "fp64"), -1)  # the library rejects -1
+        L.SWESim_set_precision(self.ptr, prec)
+        self._check("set_precision")
+        (L.SWESim_init if device else L.SWESim_init_host)(self.ptr)
+        self._check("init")
+        self.nt = L.SWESim_get_nt(self.ptr)
+        self.nb = L.SWESim_get_num_orb(self.ptr)
+        self.N = int(param["nr1"]) * int(param["nr2"])
+
+    def _check(self, what):
+        if self.lib.SWESim_get_status(self.ptr) != 0:
+            raise RuntimeError(f"SWESim_{what}: {self.lib.SWESim_get_error(self.ptr).decode()}")
+
+    def run(self):
+        self.lib.SWESim_run_simulation(self.ptr)
+        self._check("run_simulation")
+        return self.lib.SWESim_last_run_ms(self.ptr)
+
+    def kernel_ms(self):
+        return self.lib.SWESim_last_kernel_ms(self.ptr)
+
+    def current(self):
+        t = np.zeros(self.nt)
+        J = [np.zeros(self.nt, dtype=np.complex128) for _ in range(3)]
+        self.lib.SWESim_get_current(self.ptr, t, *J)
+        self._check("get_current")
+        return t, np.stack(J, axis=1)
+
+    def fields(self):
+        t = np.zeros(self.nt); E = np.zeros(3 * self.nt); A = np.zeros(3 * self.nt)
+        self.lib.SWESim_get_fields(self.ptr, t, E, A)
+        self._check("get_fields")
+        return t, E.reshape(3, self.nt), A.reshape(3, self.nt)
+
+    def rho_k(self):
+        out = np.zeros(self.N * self.nb * self.nb, dtype=np.complex128)
+        self.lib.SWESim_get_rho_k(self.ptr, out)
+        self._check("get_rho_k")
+        return out.reshape(self.N, self.nb, self.nb)
+
+    def close(self):
+        if self.ptr:
+            self.lib.SWESim_delete(self.ptr)
+            self.ptr = None
+        if self.st:
+            self.lib.Settings_delete(self.st)
+            self.st = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,82 @@
+/* hhg_swe.h -- C ABI of libwannier.so, the MI355X k-space SBE propagator.
+ *
+ * The first block is exactly the reference's C ABI (src/cwrapper.cpp:5-28),
+ * symbol for symbol, so the reference's ctypes driver (pySWE/pyswe.py:9-98)
+ * loads this library unchanged. `cdouble` is std::complex<double>, which is
+ * layout-identical to double[2] and to numpy complex128.
+ *
+ * The second block is additive: error reporting (the reference throws or
+ * calls std::exit across the ABI), precision selection and read-back hooks.
+ * Handles are opaque. Every entry point catches its own errors; a failed call
+ * leaves a message readable with SWESim_get_error / hhg_last_error.
+ */
+#ifndef HHG_SWE_H
+#define HHG_SWE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct SWESim SWESim;
+typedef struct Settings Settings;
+typedef struct { double re, im; } hhg_cdouble;  /* = std::complex<double> */
+
+/* ---- reference ABI (src/cwrapper.cpp) ---------------------------------- */
+SWESim* SWESim_new(void);                                   /* cwrapper.cpp:7  */
+void SWESim_run_simulation(SWESim* swe);                    /* cwrapper.cpp:8  */
+void SWESim_test_files(SWESim* swe);                        /* cwrapper.cpp:9  */
+void SWESim_set_path_tb(SWESim* swe, char* path_tb);        /* cwrapper.cpp:10 */
+void SWESim_set_settings(SWESim* swe, Settings* settings);  /* cwrapper.cpp:11 */
+void SWESim_init(SWESim* swe);                              /* cwrapper.cpp:12 */
+void SWESim_restart(SWESim* swe);                           /* cwrapper.cpp:13 */
+void SWESim_get_current(SWESim* swe, double* t, hhg_cdouble* jx,
+                        hhg_cdouble* jy, hhg_cdouble* jz);  /* cwrapper.cpp:14 */
+void SWESim_delete(SWESim* swe);                            /* cwrapper.cpp:15 */
+
+Settings* Settings_new(void);                               /* cwrapper.cpp:17 */
+void Settings_set_nr1(Settings* s, int val);                /* cwrapper.cpp:18 */
+void Settings_set_nr2(Settings* s, int val);                /* cwrapper.cpp:19 */
+void Settings_set_nr3(Settings* s, int val);                /* cwrapper.cpp:20 */
+void Settings_set_tmax(Settings* s, double val);            /* cwrapper.cpp:21 */
+void Settings_set_dt(Settings* s, double val);              /* cwrapper.cpp:22 */
+void Settings_set_intensity(Settings* s, double val);       /* cwrapper.cpp:23 */
+void Settings_set_lambda(Settings* s, double val);          /* cwrapper.cpp:24 */
+void Settings_set_tmax_field(Settings* s, double val);      /* cwrapper.cpp:25 */
+void Settings_set_pol_vec(Settings* s, double* val);        /* cwrapper.cpp:26, val[3] */
+void Settings_set_phi_vec(Settings* s, double* val);        /* cwrapper.cpp:27, val[3] */
+
+/* ---- extensions (no counterpart in the reference) ----------------------- */
+/* SWESim_init without touching the GPU: parses the model and builds the
+ * host tables (fields, support, initial state). run_simulation then fails
+ * until SWESim_init is called. */
+void SWESim_init_host(SWESim* swe);
+/* Frees a Settings block (the reference has no such call and leaks it). */
+void Settings_delete(Settings* s);
+/* 0 = last call on this handle succeeded; otherwise an error code. */
+int SWESim_get_status(SWESim* swe);
+/* Message of the last failed call on this handle ("" when none). */
+const char* SWESim_get_error(SWESim* swe);
+/* Message of the last failed call in this thread, on any handle. */
+const char* hhg_last_error(void);
+/* Number of time steps of the initialised solve (= (int)(tmax/dt)). */
+int SWESim_get_nt(SWESim* swe);
+int SWESim_get_num_orb(SWESim* swe);
+/* Arithmetic of the RK4 stages: 0 = FP64 (default), 1 = FP32 stages with the
+ * band basis, initial state, state update and J accumulation in FP64. */
+void SWESim_set_precision(SWESim* swe, int precision);
+/* Field tables exactly as the time loop consumes them: t[nt] (au),
+ * E[3][nt], A[3][nt]. Returns 0 on success. */
+int SWESim_get_fields(SWESim* swe, double* t, double* E, double* A);
+/* Current k-space density matrices rho_k, [nr1*nr2][nb][nb], k = m1*nr2+m2
+ * (after SWESim_init_host: the initial state). Returns 0 on success. */
+int SWESim_get_rho_k(SWESim* swe, hhg_cdouble* rho);
+/* Device time of the last SWESim_run_simulation in milliseconds (HIP
+ * events around the propagation kernels, J reduction included). */
+double SWESim_last_run_ms(SWESim* swe);
+/* Device time of the propagation kernel alone in the last run (ms). */
+double SWESim_last_kernel_ms(SWESim* swe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HHG_SWE_H */

@@ -1,0 +1,126 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Tolerances: FP64 path <= 1e-10 relative L2 on J(t) (the target for the plain
+path); invariants of the exact dynamics (trace, Hermiticity) to rounding.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, HALDANE, HMCASE0, make_param, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def native():
+    from hhgamd import lib
+    return lib
+
+
+def solve(native, prm):
+    with native.Solve(prm) as s:
+        s.run()
+        t, J = s.current()
+        rho = s.rho_k()
+    return t, J, rho
+
+
+@pytest.mark.parametrize("name,path,n", [("hmcase0", HMCASE0, 10), ("hmcase0", HMCASE0, 30), ("haldane", HALDANE, 30)])
+def test_fp64_vs_golden(native, name, path, n):
+    g = np.load(f"{GOLDEN}/golden_{name}_{n}.npz")
+    t, J, _ = solve(native, make_param(path, nr=n))
+    assert np.array_equal(t, g["t"])
+    err = rel_l2(J, g["J"])
+    print(f"{name} {n}x{n}: rel L2 vs golden = {err:.3e}")
+    assert err <= FP64_TOL
+
+
+def test_pyswe_readme_flow(native):
+    """scripts/test_swe.py call sequence through the pySWE mirror (30x30)."""
+    from pySWE.pyswe import Settings, SWE
+    prm = make_param(HMCASE0, nr=30)
+    settings = Settings(prm)
+    swe = SWE(settings)
+    swe.run_simulation()
+    t, jx, jy, jz = swe.get_current()
+    swe.delete()
+    g = np.load(f"{GOLDEN}/golden_hmcase0_30.npz")
+    J = np.stack([jx, jy, jz], axis=1)
+    assert np.array_equal(t.real, g["t"])
+    assert rel_l2(J, g["J"]) <= FP64_TOL
+
+
+def test_c2_haldane_200_full_vs_oracle(native, oracle_lib):
+    """200x200 k-grid, 4096 RK4 steps, pol along y: the plain FP64 workload."""
+    prm = make_param(HALDANE, nr=200)
+    t, J, rho = solve(native, prm)
+    t2, J2 = oracle_lib.run(HALDANE, prm, mode="perk", nthreads=min(16, os.cpu_count() or 1))
+    assert np.array_equal(t, t2)
+    err = rel_l2(J, J2)
+    print(f"C2 rel L2 vs oracle = {err:.3e}")
+    assert err <= FP64_TOL
+    tr = np.trace(rho, axis1=1, axis2=2)
+    assert np.abs(tr - 1.0).max() < 1e-12
+    assert np.abs(rho - rho.conj().transpose(0, 2, 1)).max() < 1e-12
+
+
+def test_oblique_polarisation_with_phases(native, oracle_lib):
+    prm = make_param(HMCASE0, nr=(12, 16), tmax=40.0, tmax_field=35.0, intensity=4e12,
+                     pol=(0.96, 0.25, 0.0), phi=(0.0, 0.7, 0.0))
+    _, J, _ = solve(native, prm)
+    _, J2 = oracle_lib.run(HMCASE0, prm, mode="perk")
+    assert rel_l2(J, J2) <= FP64_TOL
+
+
+def test_field_free_is_stationary(native):
+    prm = make_param(HALDANE, nr=16, tmax=10.0, intensity=0.0)
+    with native.Solve(prm, device=False) as s:
+        rho0 = s.rho_k()
+    _, J, rho = solve(native, prm)
+    assert np.abs(J - J[0]).max() < 1e-15
+    assert np.abs(rho - rho0).max() < 1e-13
+
+
+@pytest.mark.parametrize("nb,nr,tmax", [(3, 8, 6.0), (4, 10, 10.0), (8, 12, 10.0), (16, 6, 3.0)])
+def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax):
+    from hhgamd import tbgen
+    path = str(tmp_path / f"syn{nb}_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=nb))
+    prm = make_param(path, nr=nr, tmax=tmax, tmax_field=tmax * 0.8, intensity=5e11, pol=(0.6, 0.8, 0.0))
+    _, J, rho = solve(native, prm)
+    _, J2 = oracle_lib.run(path, prm, mode="perk")
+    err = rel_l2(J, J2)
+    print(f"nb={nb}: rel L2 vs oracle = {err:.3e}")
+    assert err <= FP64_TOL
+    assert np.abs(np.trace(rho, axis1=1, axis2=2) - np.trace(rho, axis1=1, axis2=2).mean()).max() < 1e-10
+
+
+def test_run_twice_continues_and_keeps_first_samples(native):
+    """src/swe.cpp:183-223 + src/observable.cpp:29: a second run continues from
+    the evolved state and appends; get_current still returns the first nt."""
+    prm = make_param(HALDANE, nr=8, tmax=5.0)
+    with native.Solve(prm) as s:
+        s.run()
+        _, J1 = s.current()
+        rho1 = s.rho_k()
+        s.run()
+        _, J1b = s.current()
+        rho2 = s.rho_k()
+    assert np.array_equal(J1, J1b)
+    assert np.abs(rho2 - rho1).max() > 0
+
+
+def test_restart_reinitialises(native):
+    prm = make_param(HALDANE, nr=8, tmax=5.0)
+    with native.Solve(prm) as s:
+        s.run()
+        _, J1 = s.current()
+        s.lib.SWESim_restart(s.ptr)
+        s._check("restart")
+        s.run()
+        _, J2 = s.current()
+    assert np.array_equal(J1, J2)
