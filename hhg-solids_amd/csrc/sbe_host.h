@@ -41,6 +41,10 @@ struct HostSolve {
     std::vector<double> tw1, tw2;    // N1, N2 complex
     std::vector<double> rho0;        // nb*nb*N complex, element-major
     std::vector<double> t, E[3], A[3];
+    // Hermitian nb = 2 pack (valid when pd.nrep > 0)
+    PairDesc pd{};
+    std::vector<double> pcoef;       // PAIR_HEAD + PAIR_MAX*PAIR_REP
+    std::vector<double> psteps;      // nt * pd.stride
 };
 HostSolve build_solve(const TBModel& m, const SolveParams& p);
 
@@ -53,6 +57,10 @@ double dev_run(DeviceSolve* ds, int precision, double* J_out);
 void dev_get_rho(DeviceSolve* ds, double* rho_elem_major);
 // Device time of the propagation kernel alone in the last dev_run.
 double dev_last_kernel_ms(DeviceSolve* ds);
+// 0 = automatic (Hermitian-paired kernel when the blocks allow), 1 = generic.
+void dev_set_variant(DeviceSolve* ds, int variant);
+// Kernel the next run launches: 1 thread-per-k, 2 Hermitian nb=2, 3 element-per-lane.
+int dev_kernel_variant(DeviceSolve* ds);
 void dev_destroy(DeviceSolve* ds);
 
 }  // namespace hhg

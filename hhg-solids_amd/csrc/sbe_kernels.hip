@@ -12,6 +12,9 @@
 //   J_a  += sum_s w_s sum_ij v_a,s[ij] conj(rho_ij)       (4x4 window, src/observable.cpp:16-30)
 //
 // Kernels:
+//   k_pair2<NREP,NX,NV,T>      nb = 2 with Hermitian blocks: one lane per k,
+//                              the +-R pairs folded into real coefficient
+//                              matrices, branch-free time loop.
 //   k_thread_per_k<NB,MAXU,T>  one lane per k, everything in registers; the
 //                              blocks and the per-step table are uniform and
 //                              arrive through scalar loads. For nb <= 2.
@@ -68,6 +71,9 @@ __device__ __forceinline__ cx<T> ld(const double* p) {  // complex from interlea
     return {(T)p[0], (T)p[1]};
 }
 
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -78,6 +84,38 @@ __device__ __forceinline__ int posmod(long a, int n) {
     int r = (int)(a % n);
     return r < 0 ? r + n : r;
 }
+
+// Deferred per-wave reduction of the current. Each lane parks its C per-step
+// components for TCH steps in an LDS row (padded to avoid bank conflicts);
+// every TCH steps lane p sums component p of the chunk over the 64 rows in a
+// fixed order and writes the wave partial. Replaces a shuffle tree per step.
+template <int C, int TCH>
+struct JStage {
+    static_assert(C * TCH <= 64, "one lane per (step, component) of a chunk");
+    static constexpr int ROW = C * TCH + 1;
+    static constexpr size_t BYTES = sizeof(double) * 64 * ROW;
+    __device__ static void put(double* lds, int lane, int slot, const double* v) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) lds[lane * ROW + slot * C + c] = v[c];
+    }
+    // steps t0 .. t0+nsteps-1 are parked; jpart is [nt][nwaves][C]
+    __device__ static void flush(const double* lds, int lane, int t0, int nsteps, double* jpart, int nwaves,
+                                 int wave) {
+        __syncthreads();
+        if (lane < C * nsteps) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll 4
+            for (int l = 0; l < 64; l += 4) {
+                a0 += lds[l * ROW + lane];
+                a1 += lds[(l + 1) * ROW + lane];
+                a2 += lds[(l + 2) * ROW + lane];
+                a3 += lds[(l + 3) * ROW + lane];
+            }
+            jpart[((size_t)(t0 + lane / C) * nwaves + wave) * C + lane % C] = (a0 + a1) + (a2 + a3);
+        }
+        __syncthreads();
+    }
+};
 
 // --------------------------------------------------------------------------
 // One lane per k. MAXU bounds the support size so that the per-k DFT phases
@@ -116,6 +154,9 @@ __global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const do
     const T half = (T)(0.5 * dt);
     const T full = (T)dt;
     const double pref = 0.1666666666666 * dt;
+    constexpr int TCH = 8;
+    using JS = JStage<6, TCH>;
+    __shared__ double jl[64 * JS::ROW];
 
     for (int ti = 0; ti < d.nt; ++ti) {
         const double* st = steptab + (size_t)ti * d.stride;
@@ -224,13 +265,8 @@ __global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const do
                     }
             }
         }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) jc[c] = wave_sum(jc[c]);
-        if (lane == 0) {
-            double* out = jpart + ((size_t)ti * nwaves + wave) * 6;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = jc[c];
-        }
+        JS::put(jl, lane, ti % TCH, jc);
+        if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
     }
     if (active) {
 #pragma unroll
@@ -238,6 +274,180 @@ __global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const do
             rho_out[2 * ((size_t)e * d.N + k)] = rho[e].re;
             rho_out[2 * ((size_t)e * d.N + k) + 1] = rho[e].im;
         }
+    }
+}
+
+// --------------------------------------------------------------------------
+// nb = 2 with Hermitian blocks (B(-R) = B(R)^dagger for every operator): one
+// lane per k, rho kept as (p = rho00, q = rho11, c = rho01), rho10 = conj(c).
+// Each +-R pair enters through its 4x2 real coefficient matrix (PairDesc):
+// 8 FMAs per operator and pair and step, no branches. The commutator of two
+// Hermitian 2x2 matrices H = [a b; b* d] and rho is
+//   K00 = -K11 = 2 Im(b c*),   K01 = -i ((a - d) c + (q - p) b),
+// and the current is real: J = m0 p + m1 q + m2 Re(c) + m3 Im(c).
+// The H coefficients of NREP pairs stay in registers for the whole time
+// loop; the rbc and v coefficients are read from LDS (broadcast reads).
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_pair2(const SolveDesc d, const PairDesc pd, const double* __restrict__ coef,
+                                              const double* __restrict__ steps, const double* __restrict__ tw1,
+                                              const double* __restrict__ tw2, const double* __restrict__ rho_in,
+                                              double* __restrict__ rho_out, double* __restrict__ jpart, int nwaves,
+                                              double dt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = k >> 6;
+    const int lane = threadIdx.x & 63;
+    constexpr int TCH = 16;
+    using JS = JStage<3, TCH>;
+    __shared__ double jl[64 * JS::ROW];
+    constexpr int NXA = NX > 0 ? NX : 1;
+
+    // All coefficients in VGPRs for the whole time loop. They are uniform, so
+    // the compiler would hold them in SGPRs and spill; adding 0.0 * lane
+    // (exact for finite values) makes each copy lane-varying.
+    const double lz = 0.0 * (double)lane;
+    cx<double> w0[NREP];
+    T hc[NREP][8], xc[NREP][NXA][8];
+    double vc[NREP][NV][8];
+#pragma unroll
+    for (int r = 0; r < NREP; ++r) {
+        const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
+        w0[r] = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
+        const double* cr = coef + PAIR_HEAD + PAIR_REP * r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            hc[r][j] = (T)(cr[j] + lz);
+#pragma unroll
+            for (int x = 0; x < NX; ++x) xc[r][x][j] = (T)(cr[8 + 8 * x + j] + lz);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) vc[r][v][j] = cr[32 + 8 * v + j] + lz;
+        }
+    }
+    T h0[4], x0[NXA][4];
+    double v0[NV][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        h0[j] = (T)(coef[j] + lz);
+#pragma unroll
+        for (int x = 0; x < NX; ++x) x0[x][j] = (T)(coef[4 + 4 * x + j] + lz);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) v0[v][j] = coef[16 + 4 * v + j] + lz;
+    }
+
+    double p = 0, q = 0;
+    cx<double> c = {0, 0};
+    if (active) {
+        p = rho_in[0 * 2 * (size_t)d.N + 2 * k];
+        c = ld<double>(rho_in + 1 * 2 * (size_t)d.N + 2 * k);
+        q = rho_in[3 * 2 * (size_t)d.N + 2 * k];
+    }
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    // per-step row, loaded one step ahead
+    double ph[NREP][2], ef[3][NXA];
+    auto load_row = [&](int ti, double (&a)[NREP][2], double (&e)[3][NXA]) {
+        const double* row = steps + (size_t)ti * pd.stride;
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) { a[r][0] = row[PAIR_STEP_HEAD + 2 * r]; a[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) e[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
+    };
+    load_row(0, ph, ef);
+
+    for (int ti = 0; ti < d.nt; ++ti) {
+        double phn[NREP][2], efn[3][NXA];
+        load_row(ti + 1 < d.nt ? ti + 1 : ti, phn, efn);
+
+        cx<double> w[NREP];
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) w[r] = cmul(w0[r], cx<double>{ph[r][0], ph[r][1]});
+        // field-free part of H^(k - A): four real outputs
+        T hs[4] = {h0[0], h0[1], h0[2], h0[3]};
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const T wr = (T)w[r].re, wi = (T)w[r].im;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, hs[j]));
+        }
+        T xs[NXA][4];
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xs[x][j] = x0[x][j];
+#pragma unroll
+            for (int r = 0; r < NREP; ++r) {
+                const T wr = (T)w[r].re, wi = (T)w[r].im;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, xs[x][j]));
+            }
+        }
+        T A, D, Br, Bi;
+        auto stage_h = [&](int j) {
+            A = hs[0]; D = hs[1]; Br = hs[2]; Bi = hs[3];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const T f = (T)ef[j][x];
+                A = fmaT(f, xs[x][0], A); D = fmaT(f, xs[x][1], D);
+                Br = fmaT(f, xs[x][2], Br); Bi = fmaT(f, xs[x][3], Bi);
+            }
+        };
+        T kp, kr, ki;
+        auto deriv = [&](T sp, T sq, T scr, T sci) {
+            kp = (T)2 * fmaT(Bi, scr, -(Br * sci));
+            const T ad = A - D, qp = sq - sp;
+            kr = fmaT(ad, sci, qp * Bi);
+            ki = -fmaT(ad, scr, qp * Br);
+        };
+        const T P = (T)p, Q = (T)q, Cr = (T)c.re, Ci = (T)c.im;
+        stage_h(0);
+        deriv(P, Q, Cr, Ci);
+        T ap = kp, ar = kr, ai = ki;
+        stage_h(1);
+        deriv(fmaT(half, kp, P), fmaT(-half, kp, Q), fmaT(half, kr, Cr), fmaT(half, ki, Ci));
+        ap = fmaT((T)2, kp, ap); ar = fmaT((T)2, kr, ar); ai = fmaT((T)2, ki, ai);
+        deriv(fmaT(half, kp, P), fmaT(-half, kp, Q), fmaT(half, kr, Cr), fmaT(half, ki, Ci));
+        ap = fmaT((T)2, kp, ap); ar = fmaT((T)2, kr, ar); ai = fmaT((T)2, ki, ai);
+        stage_h(2);
+        deriv(fmaT(full, kp, P), fmaT(-full, kp, Q), fmaT(full, kr, Cr), fmaT(full, ki, Ci));
+        ap += kp; ar += kr; ai += ki;
+        p = fmaT(pref, (double)ap, p);
+        q = fmaT(-pref, (double)ap, q);
+        c.re = fmaT(pref, (double)ar, c.re);
+        c.im = fmaT(pref, (double)ai, c.im);
+        // current of the new state with the phases of step ti, FP64
+        double jc[3] = {0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double m[4] = {v0[v][0], v0[v][1], v0[v][2], v0[v][3]};
+#pragma unroll
+            for (int r = 0; r < NREP; ++r)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) m[j] = fmaT(vc[r][v][2 * j + 1], w[r].im, fmaT(vc[r][v][2 * j], w[r].re, m[j]));
+            // inactive lanes hold rho = 0 and contribute exactly 0
+            jc[v] = fmaT(m[3], c.im, fmaT(m[2], c.re, fmaT(m[1], q, m[0] * p)));
+        }
+        JS::put(jl, lane, ti % TCH, jc);
+        if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) { ph[r][0] = phn[r][0]; ph[r][1] = phn[r][1]; }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) ef[j][x] = efn[j][x];
+    }
+    if (active) {
+        double* o = rho_out + 2 * (size_t)k;
+        const size_t st = 2 * (size_t)d.N;
+        o[0] = p; o[1] = 0.0;
+        o[st] = c.re; o[st + 1] = c.im;
+        o[2 * st] = c.re; o[2 * st + 1] = -c.im;
+        o[3 * st] = q; o[3 * st + 1] = 0.0;
     }
 }
 
@@ -420,22 +630,34 @@ __global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const 
         }
 }
 
-// J[ti][c] = (sum over waves, in wave order per lane stripe) / N
-__global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, double N,
-                                                 double* __restrict__ J) {
+// J[ti][c] = (sum over waves, in wave order per lane stripe) / N. The
+// partials carry C components per wave: C = 6 (re/im of x, y, z) or C = 3
+// (real parts of the current along the axes ax0, ax1, ax2 -- the Hermitian
+// path, whose current is real; unused slots are zero).
+__global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, int C, int ax0,
+                                                 int ax1, int ax2, double N, double* __restrict__ J) {
     const int ti = blockIdx.x;
     const int lane = threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int w = lane; w < nwaves; w += 64) {
-        const double* p = jpart + ((size_t)ti * nwaves + w) * 6;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] += p[c];
+        const double* p = jpart + ((size_t)ti * nwaves + w) * C;
+        for (int c = 0; c < C; ++c) acc[c] += p[c];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
     if (lane == 0) {
+        double* out = J + (size_t)ti * 6;
+        if (C == 6) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) J[(size_t)ti * 6 + c] = acc[c] / N;
+            for (int c = 0; c < 6; ++c) out[c] = acc[c] / N;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = 0.0;
+            const int ax[3] = {ax0, ax1, ax2};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (ax[c] >= 0) out[2 * ax[c]] = acc[c] / N;
+        }
     }
 }
 
@@ -445,9 +667,12 @@ struct DeviceSolve {
     double dt = 0;
     double *blk = nullptr, *steptab = nullptr, *tw1 = nullptr, *tw2 = nullptr;
     double *rho = nullptr, *rho_tmp = nullptr, *jpart = nullptr, *J = nullptr;
+    PairDesc pd{};
+    double *pcoef = nullptr, *psteps = nullptr;
     size_t jpart_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
+    bool use_herm = true;  // paired Hermitian kernel when the blocks allow it
     float last_kernel_ms = 0.f;
 };
 
@@ -473,6 +698,11 @@ DeviceSolve* dev_create(const HostSolve& hs) {
         ds->tw1 = upload(hs.tw1, ds->stream);
         ds->tw2 = upload(hs.tw2, ds->stream);
         ds->rho = upload(hs.rho0, ds->stream);
+        ds->pd = hs.pd;
+        if (hs.pd.nrep > 0) {
+            ds->pcoef = upload(hs.pcoef, ds->stream);
+            ds->psteps = upload(hs.psteps, ds->stream);
+        }
         HIP_CHECK(hipMalloc(&ds->rho_tmp, sizeof(double) * hs.rho0.size()));
         HIP_CHECK(hipMalloc(&ds->J, sizeof(double) * 6 * (size_t)hs.d.nt));
         HIP_CHECK(hipStreamSynchronize(ds->stream));
@@ -485,7 +715,8 @@ DeviceSolve* dev_create(const HostSolve& hs) {
 
 void dev_destroy(DeviceSolve* ds) {
     if (!ds) return;
-    for (double* p : {ds->blk, ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, ds->J})
+    for (double* p : {ds->blk, ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, ds->J, ds->pcoef,
+                      ds->psteps})
         if (p) (void)hipFree(p);
     if (ds->ev0) (void)hipEventDestroy(ds->ev0);
     if (ds->ev1) (void)hipEventDestroy(ds->ev1);
@@ -500,8 +731,8 @@ static int pow2ceil(int n) {
     return g;
 }
 
-static void ensure_jpart(DeviceSolve* ds, int nwaves) {
-    size_t need = sizeof(double) * 6 * (size_t)ds->d.nt * nwaves;
+static void ensure_jpart(DeviceSolve* ds, int nwaves, int C = 6) {
+    size_t need = sizeof(double) * C * (size_t)ds->d.nt * nwaves;
     if (need > ds->jpart_bytes) {
         if (ds->jpart) HIP_CHECK(hipFree(ds->jpart));
         ds->jpart = nullptr;
@@ -543,15 +774,43 @@ static void launch_threads(DeviceSolve* ds, int& nwaves) {
     HIP_CHECK(hipGetLastError());
 }
 
-template <typename T>
-static void launch_any(DeviceSolve* ds, int& nwaves) {
+template <int NREP, int NX, int NV, typename T>
+static void launch_pair2(DeviceSolve* ds, int& nwaves) {
     const SolveDesc& d = ds->d;
-    if (d.nb == 1 && d.ns <= 16) return launch_threads<1, 16, T>(ds, nwaves);
-    if (d.nb == 2 && d.ns <= 8) return launch_threads<2, 8, T>(ds, nwaves);
-    if (d.nb == 2 && d.ns <= 16) return launch_threads<2, 16, T>(ds, nwaves);
-    if (d.nb == 2 && d.ns <= 32) return launch_threads<2, 32, T>(ds, nwaves);
-    if (d.nb * d.nb <= 64) return launch_elems<1, T>(ds, nwaves);
-    if (d.nb <= 16) return launch_elems<4, T>(ds, nwaves);
+    const int block = 64;
+    const int nblk = (d.N + block - 1) / block;
+    nwaves = nblk;
+    ensure_jpart(ds, nwaves, 3);
+    hipLaunchKernelGGL((k_pair2<NREP, NX, NV, T>), dim3(nblk), dim3(block), 0, ds->stream, d, ds->pd, ds->pcoef,
+                       ds->psteps, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int NREP, typename T>
+static void pair2_nx_nv(DeviceSolve* ds, int& nwaves) {
+    const PairDesc& pd = ds->pd;
+    if (pd.nx == 0 && pd.nv <= 2) return launch_pair2<NREP, 0, 2, T>(ds, nwaves);
+    if (pd.nx == 0) return launch_pair2<NREP, 0, 3, T>(ds, nwaves);
+    if (pd.nx <= 2 && pd.nv <= 2) return launch_pair2<NREP, 2, 2, T>(ds, nwaves);
+    return launch_pair2<NREP, 3, 3, T>(ds, nwaves);
+}
+
+// Returns the number of current components per wave partial (6 or 3).
+template <typename T>
+static int launch_any(DeviceSolve* ds, int& nwaves) {
+    const SolveDesc& d = ds->d;
+    if (ds->pd.nrep > 0 && ds->use_herm) {
+        if (ds->pd.nrep <= 3) return pair2_nx_nv<3, T>(ds, nwaves), 3;
+        if (ds->pd.nrep <= 4) return pair2_nx_nv<4, T>(ds, nwaves), 3;
+        if (ds->pd.nrep <= 8) return pair2_nx_nv<8, T>(ds, nwaves), 3;
+        return pair2_nx_nv<PAIR_MAX, T>(ds, nwaves), 3;
+    }
+    if (d.nb == 1 && d.ns <= 16) return launch_threads<1, 16, T>(ds, nwaves), 6;
+    if (d.nb == 2 && d.ns <= 8) return launch_threads<2, 8, T>(ds, nwaves), 6;
+    if (d.nb == 2 && d.ns <= 16) return launch_threads<2, 16, T>(ds, nwaves), 6;
+    if (d.nb == 2 && d.ns <= 32) return launch_threads<2, 32, T>(ds, nwaves), 6;
+    if (d.nb * d.nb <= 64) return launch_elems<1, T>(ds, nwaves), 6;
+    if (d.nb <= 16) return launch_elems<4, T>(ds, nwaves), 6;
     throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
 }
 
@@ -559,12 +818,12 @@ double dev_run(DeviceSolve* ds, int precision, double* J_out) {
     const SolveDesc& d = ds->d;
     int nwaves = 0;
     HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
-    if (precision == 0)
-        launch_any<double>(ds, nwaves);
-    else
-        launch_any<float>(ds, nwaves);
+    const int C = precision == 0 ? launch_any<double>(ds, nwaves) : launch_any<float>(ds, nwaves);
     HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
-    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, (double)d.N, ds->J);
+    int ax[3] = {-1, -1, -1};
+    for (int v = 0; v < ds->pd.nv && C == 3; ++v) ax[v] = ds->pd.vaxis[v];
+    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, C, ax[0], ax[1], ax[2],
+                       (double)d.N, ds->J);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(ds->ev1, ds->stream));
     std::swap(ds->rho, ds->rho_tmp);
@@ -574,6 +833,15 @@ double dev_run(DeviceSolve* ds, int precision, double* J_out) {
     HIP_CHECK(hipEventElapsedTime(&ms, ds->ev0, ds->ev1));
     HIP_CHECK(hipEventElapsedTime(&ds->last_kernel_ms, ds->ev0, ds->evk));
     return ms;
+}
+
+void dev_set_variant(DeviceSolve* ds, int variant) { ds->use_herm = (variant == 0); }
+
+int dev_kernel_variant(DeviceSolve* ds) {
+    const SolveDesc& d = ds->d;
+    if (ds->pd.nrep > 0 && ds->use_herm) return 2;                    // k_pair2
+    if (d.nb <= 2 && d.ns <= 32) return 1;                           // k_thread_per_k
+    return 3;                                                         // k_elem_per_lane
 }
 
 double dev_last_kernel_ms(DeviceSolve* ds) { return ds ? ds->last_kernel_ms : 0.0; }

@@ -18,6 +18,7 @@
 //   Peierls phase     src/swe.cpp:225-233
 //   initial state     src/swe.cpp:80-90, src/hamiltonian.cpp:70-101,
 //                     src/operator.cpp:165-280, src/rdm.cpp:23-38
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -292,12 +293,44 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
             }
         }
     if ((int)sup.size() > MAX_SUPPORT) throw std::runtime_error("model support exceeds MAX_SUPPORT points");
+    // rbc axes only count as live when some block is non-zero
+    {
+        int xm = 0;
+        for (auto& q : sup)
+            for (int ax = 0; ax < 3; ++ax)
+                if (q.om & (1u << (OP_X + ax))) xm |= 1 << ax;
+        d.xmask &= xm;
+    }
+    // Hermitian pairing: B(-n) = B(n)^dagger for every op (to 1e-13 of the
+    // block scale, which covers the rounding of the convolved velocity) and
+    // every partner inside the supercell window the op lives on.
+    d.herm = 1;
+    for (size_t i = 0; i < sup.size() && d.herm; ++i) {
+        const Sup& a = sup[i];
+        const Sup* b = nullptr;
+        for (auto& q : sup) if (q.n1 == -a.n1 && q.n2 == -a.n2) b = &q;
+        for (int o = 0; o < NOPS && d.herm; ++o) {
+            const cplx* A = &a.b[(size_t)o * nb2];
+            double scale = 0.0;
+            for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(A[e]));
+            if (scale == 0.0 && !(b && (b->om & (1u << o)))) continue;
+            if (!b) { d.herm = 0; break; }
+            const cplx* B = &b->b[(size_t)o * nb2];
+            for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(B[e]));
+            for (int i1 = 0; i1 < nb; ++i1)
+                for (int j1 = 0; j1 < nb; ++j1)
+                    if (std::abs(B[j1 * nb + i1] - std::conj(A[i1 * nb + j1])) > 1e-13 * scale) d.herm = 0;
+        }
+    }
     const int ns = (int)sup.size();
     d.ns = ns;
     d.stride = STEP_HEAD + 2 * ns;
     hs.blk.assign((size_t)ns * NOPS * nb2 * 2, 0.0);
     for (int s = 0; s < ns; ++s) {
         d.n1[s] = sup[s].n1; d.n2[s] = sup[s].n2; d.opmask[s] = sup[s].om;
+        const bool origin = sup[s].n1 == 0 && sup[s].n2 == 0;
+        const bool rep = sup[s].n1 > 0 || (sup[s].n1 == 0 && sup[s].n2 > 0);
+        d.role[s] = origin ? ROLE_ORIGIN : (rep ? ROLE_REP : ROLE_PARTNER);
         std::memcpy(&hs.blk[(size_t)s * NOPS * nb2 * 2], sup[s].b.data(), sizeof(cplx) * NOPS * nb2);
     }
 
@@ -322,6 +355,63 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
             cplx ph = std::exp(cplx(0.0, +ax_ * Rb[3 * s] + ay_ * Rb[3 * s + 1] + az_ * Rb[3 * s + 2]));
             row[STEP_HEAD + 2 * s] = ph.real();
             row[STEP_HEAD + 2 * s + 1] = ph.imag();
+        }
+    }
+
+    // Pair pack for the Hermitian nb = 2 kernel.
+    hs.pd = PairDesc{};
+    if (nb == 2 && d.herm) {
+        PairDesc& pd = hs.pd;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (d.xmask & (1 << ax)) pd.xaxis[pd.nx++] = ax;
+            if (d.vmask & (1 << ax)) pd.vaxis[pd.nv++] = ax;
+        }
+        std::vector<int> reps;
+        for (int q = 0; q < ns; ++q)
+            if (d.role[q] == ROLE_REP) reps.push_back(q);
+        if ((int)reps.size() <= PAIR_MAX) {
+            pd.nrep = (int)reps.size();
+            hs.pcoef.assign(PAIR_HEAD + PAIR_MAX * PAIR_REP, 0.0);
+            auto blk_of = [&](int q, int o) { return &sup[q].b[(size_t)o * 4]; };
+            auto origin = [&](int o, double* out, double scale01) {
+                for (int q = 0; q < ns; ++q)
+                    if (d.role[q] == ROLE_ORIGIN) {
+                        const cplx* B = blk_of(q, o);
+                        out[0] = B[0].real(); out[1] = B[3].real();
+                        out[2] = scale01 * B[1].real(); out[3] = scale01 * B[1].imag();
+                    }
+            };
+            auto pair = [&](int q, int o, double* out, double scale01) {
+                const cplx* B = blk_of(q, o);
+                const cplx b00 = B[0], b01 = B[1], b10 = B[2], b11 = B[3];
+                out[0] = 2 * b00.real(); out[1] = -2 * b00.imag();
+                out[2] = 2 * b11.real(); out[3] = -2 * b11.imag();
+                out[4] = scale01 * (b01.real() + b10.real()); out[5] = scale01 * (-b01.imag() - b10.imag());
+                out[6] = scale01 * (b01.imag() - b10.imag()); out[7] = scale01 * (b01.real() - b10.real());
+            };
+            double* c = hs.pcoef.data();
+            origin(OP_H, c, 1.0);
+            for (int x = 0; x < pd.nx; ++x) origin(OP_X + pd.xaxis[x], c + 4 + 4 * x, 1.0);
+            for (int v = 0; v < pd.nv; ++v) origin(OP_V + pd.vaxis[v], c + 16 + 4 * v, 2.0);
+            for (int r = 0; r < pd.nrep; ++r) {
+                double* cr = c + PAIR_HEAD + PAIR_REP * r;
+                pd.n1[r] = sup[reps[r]].n1; pd.n2[r] = sup[reps[r]].n2;
+                pair(reps[r], OP_H, cr, 1.0);
+                for (int x = 0; x < pd.nx; ++x) pair(reps[r], OP_X + pd.xaxis[x], cr + 8 + 8 * x, 1.0);
+                for (int v = 0; v < pd.nv; ++v) pair(reps[r], OP_V + pd.vaxis[v], cr + 32 + 8 * v, 2.0);
+            }
+            pd.stride = PAIR_STEP_HEAD + 2 * PAIR_MAX;  // rows padded: unused pairs read phase 0
+            hs.psteps.assign((size_t)nt * pd.stride, 0.0);
+            for (int ti = 0; ti < nt; ++ti) {
+                const double* row = &hs.steptab[(size_t)ti * d.stride];
+                double* out = &hs.psteps[(size_t)ti * pd.stride];
+                for (int j = 0; j < 3; ++j)
+                    for (int x = 0; x < pd.nx; ++x) out[3 * j + x] = row[3 * j + pd.xaxis[x]];
+                for (int r = 0; r < pd.nrep; ++r) {
+                    out[PAIR_STEP_HEAD + 2 * r] = row[STEP_HEAD + 2 * reps[r]];
+                    out[PAIR_STEP_HEAD + 2 * r + 1] = row[STEP_HEAD + 2 * reps[r] + 1];
+                }
+            }
         }
     }
 

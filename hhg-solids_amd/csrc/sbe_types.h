@@ -22,12 +22,43 @@ enum { OP_H = 0, OP_X = 1, OP_V = 4, NOPS = 7 };
 constexpr int MAX_SUPPORT = 64;   // support points per instance (5x5 shell + window fits)
 constexpr int STEP_HEAD = 12;     // doubles before the phases in a steptab row
 
+// Role of a support point in the Hermitian-paired sums: when every block
+// satisfies B(-n) = B(n)^dagger, sum_s w_s B_s = B_0 + sum_reps [w B + conj(w) B^dagger]
+// and only the origin and one representative of each +-n pair are visited.
+enum { ROLE_ORIGIN = 0, ROLE_REP = 1, ROLE_PARTNER = 2 };
+
 struct SolveDesc {
     int nb, N1, N2, N, nt, ns;
     int stride;                   // doubles per steptab row = STEP_HEAD + 2*ns
     int xmask, vmask;             // which rbc / v axes are live
+    int herm;                     // 1 when the paired (Hermitian) sums are exact
     int n1[MAX_SUPPORT], n2[MAX_SUPPORT];
     uint32_t opmask[MAX_SUPPORT]; // bit o set when block (s, o) is non-zero
+    int role[MAX_SUPPORT];        // ROLE_* (meaningful when herm == 1)
+};
+
+// Pair pack of the Hermitian nb = 2 kernel. A +-R pair of blocks B, B^dagger
+// with phases w, conj(w) adds to the Hermitian 2x2 sum [h00 h01; h01* h11]
+//   h00 += 2 Re(w B00), h11 += 2 Re(w B11), h01 += w B01 + conj(w B10),
+// i.e. four real outputs that are linear in (Re w, Im w): 8 coefficients per
+// operator and pair. coef layout (doubles):
+//   [0, 4)                 origin H: (B00.re, B11.re, B01.re, B01.im)
+//   [4 + 4x, 8 + 4x)       origin rbc, slot x < 3
+//   [16 + 4v, 20 + 4v)     origin v, slot v < 3, the 01 entries doubled
+//   PAIR_HEAD + PAIR_REP r: per pair r: H (8), rbc slots (3 x 8), v slots (3 x 8)
+// Coefficient c = 2*j + {0: Re w, 1: Im w} of output j. The v outputs are
+// pre-scaled so that J = m0 p + m1 q + m2 Re(c) + m3 Im(c) with rho =
+// [p c; c* q]. steps rows: 9 stage fields (3 stages x 3 rbc slots), 1 pad,
+// then e^{i A(t_i).R_r} per pair r (complex).
+constexpr int PAIR_MAX = 16;
+constexpr int PAIR_HEAD = 28;
+constexpr int PAIR_REP = 56;
+constexpr int PAIR_STEP_HEAD = 10;
+
+struct PairDesc {
+    int nrep, nx, nv, stride;
+    int xaxis[3], vaxis[3];
+    int n1[PAIR_MAX], n2[PAIR_MAX];
 };
 
 }  // namespace hhg

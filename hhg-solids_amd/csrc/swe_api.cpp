@@ -38,6 +38,7 @@ struct SWESim {
     std::string path_tb;
     Settings* settings = nullptr;
     int precision = 0;
+    int variant = 0;
     std::unique_ptr<hhg::HostSolve> host;
     hhg::DeviceSolve* dev = nullptr;
     std::vector<cplx> J[3];  // appended per run, like Observable::_data
@@ -81,6 +82,7 @@ static void sim_init(SWESim* s, bool device = true) {
     p.phi[0] = st.phi_x; p.phi[1] = st.phi_y; p.phi[2] = st.phi_z;
     auto hs = std::make_unique<hhg::HostSolve>(hhg::build_solve(m, p));
     hhg::DeviceSolve* dev = device ? hhg::dev_create(*hs) : nullptr;
+    if (dev) hhg::dev_set_variant(dev, s->variant);
     hhg::dev_destroy(s->dev);
     s->dev = dev;
     s->host = std::move(hs);
@@ -194,6 +196,14 @@ API int SWESim_get_rho_k(SWESim* s, cplx* rho) {
     return s ? s->status : -1;
 }
 API double SWESim_last_run_ms(SWESim* s) { return s ? s->last_ms : 0.0; }
+API void SWESim_set_kernel_variant(SWESim* s, int v) {
+    guarded(s, [&] {
+        if (v != 0 && v != 1) throw std::runtime_error("kernel variant must be 0 (automatic) or 1 (generic)");
+        s->variant = v;
+        if (s->dev) hhg::dev_set_variant(s->dev, v);
+    });
+}
+API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev) : 0; }
 API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_last_kernel_ms(s->dev) : 0.0; }
 
 }  // extern "C"

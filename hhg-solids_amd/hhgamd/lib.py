@@ -52,6 +52,7 @@ def bind(lib):
         "SWESim_get_nt": ([V], I), "SWESim_get_num_orb": ([V], I), "SWESim_set_precision": ([V, I], None),
         "SWESim_get_fields": ([V, _D, _D, _D], I), "SWESim_get_rho_k": ([V, _C], I),
         "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
+        "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -85,6 +86,8 @@ class Solve:
         prec = {"fp64": 0, "fp32": 1}.get(param.get("precision", "fp64"), -1)  # the library rejects -1
         L.SWESim_set_precision(self.ptr, prec)
         self._check("set_precision")
+        L.SWESim_set_kernel_variant(self.ptr, {"auto": 0, "generic": 1}.get(param.get("kernel", "auto"), -1))
+        self._check("set_kernel_variant")
         (L.SWESim_init if device else L.SWESim_init_host)(self.ptr)
         self._check("init")
         self.nt = L.SWESim_get_nt(self.ptr)
@@ -99,6 +102,9 @@ class Solve:
         self.lib.SWESim_run_simulation(self.ptr)
         self._check("run_simulation")
         return self.lib.SWESim_last_run_ms(self.ptr)
+
+    def kernel(self):
+        return self.lib.SWESim_get_kernel(self.ptr)
 
     def kernel_ms(self):
         return self.lib.SWESim_last_kernel_ms(self.ptr)

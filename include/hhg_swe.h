@@ -64,6 +64,12 @@ int SWESim_get_num_orb(SWESim* swe);
 /* Arithmetic of the RK4 stages: 0 = FP64 (default), 1 = FP32 stages with the
  * band basis, initial state, state update and J accumulation in FP64. */
 void SWESim_set_precision(SWESim* swe, int precision);
+/* Kernel selection: 0 = automatic (default; the Hermitian-paired nb=2 kernel
+ * when every block satisfies B(-R) = B(R)^dagger), 1 = generic kernels. */
+void SWESim_set_kernel_variant(SWESim* swe, int variant);
+/* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2,
+ * 3 = element per lane; 0 before SWESim_init. */
+int SWESim_get_kernel(SWESim* swe);
 /* Field tables exactly as the time loop consumes them: t[nt] (au),
  * E[3][nt], A[3][nt]. Returns 0 on success. */
 int SWESim_get_fields(SWESim* swe, double* t, double* E, double* A);
