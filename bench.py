@@ -44,9 +44,11 @@ def workload_param(path_tb, nr, precision):
 
 
 def algorithmic_flops_per_kstep(nb, nR):
-    """8 (8 nb^3 + 4 nR nb^2 + 48 nb^2): 4 stages x 2 complex nb x nb products,
-    the H and three rbc Fourier sums over nR blocks, the windowed J of three
-    axes over 16 points (complex multiply-add = 8 flops)."""
+    """Generic (non-Hermitian) formulation: 8 (8 nb^3 + 4 nR nb^2 + 48 nb^2) --
+    4 stages x 2 complex nb x nb products, the H and three rbc Fourier sums
+    over nR blocks, the windowed J of three axes over 16 points (complex
+    multiply-add = 8 flops). Reported beside the launched formulation's own
+    count (SWESim_flops_per_kstep), which is what roofline.achieved uses."""
     return 8 * (8 * nb ** 3 + 4 * nR * nb ** 2 + 48 * nb ** 2)
 
 
@@ -107,6 +109,8 @@ def main():
     s = lib.Solve(prm)
     setup_s = time.perf_counter() - t_setup
     nk, nt, nb = s.N, s.nt, s.nb
+    kernel_id = s.kernel()
+    F = s.flops_per_kstep()
     for _ in range(args.warmup):
         s.run()
     lat, kern = [], []
@@ -123,7 +127,7 @@ def main():
     value = nk * nt * args.steps / wall
     kms = float(np.mean(kern))
     nR = nonzero_blocks(model[2], model[3])
-    F = algorithmic_flops_per_kstep(nb, nR)
+    F_generic = algorithmic_flops_per_kstep(nb, nR)
     achieved = F * nk * nt / (kms * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -136,8 +140,10 @@ def main():
         "roofline": {"bound": "mfma", "alu": "FP64 VALU (v_fma_f64): MI355X FP64 vector peak = FP64 matrix peak",
                      "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
-                     "kernel_ms": kms, "flops_per_kstep": F,
-                     "hbm_view_GBs": 2 * 16 * nb * nb * nk * nt / (kms * 1e-3) / 1e9},
+                     "kernel_ms": kms, "kernel": {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane"}.get(kernel_id),
+                     "flops_per_kstep": F, "generic_flops_per_kstep": F_generic,
+                     "generic_equivalent_tflops": F_generic * nk * nt / (kms * 1e-3) / 1e12,
+                     "staged_state_GBs": 2 * 16 * nb * nb * nk * nt / (kms * 1e-3) / 1e9},
     }
     prof = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(prof):

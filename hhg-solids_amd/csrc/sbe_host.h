@@ -61,6 +61,8 @@ double dev_last_kernel_ms(DeviceSolve* ds);
 void dev_set_variant(DeviceSolve* ds, int variant);
 // Kernel the next run launches: 1 thread-per-k, 2 Hermitian nb=2, 3 element-per-lane.
 int dev_kernel_variant(DeviceSolve* ds);
+// Flops per k-point and RK4 step of the formulation the next run launches.
+double dev_flops_per_kstep(DeviceSolve* ds);
 void dev_destroy(DeviceSolve* ds);
 
 }  // namespace hhg

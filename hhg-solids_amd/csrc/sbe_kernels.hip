@@ -844,6 +844,24 @@ int dev_kernel_variant(DeviceSolve* ds) {
     return 3;                                                         // k_elem_per_lane
 }
 
+// Arithmetic of the launched formulation per k-point and RK4 step, counting
+// a fused multiply-add as 2 flops. Hermitian nb = 2 kernel: pair phases
+// (6 nrep), H and rbc sums (16 nrep each), stage fields (24 nx), four
+// commutators (4 x 13), stage inputs (24), RK4 accumulation (15), update (8),
+// current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8).
+double dev_flops_per_kstep(DeviceSolve* ds) {
+    const SolveDesc& d = ds->d;
+    if (dev_kernel_variant(ds) == 2) {
+        const PairDesc& pd = ds->pd;
+        return 6.0 * pd.nrep + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 +
+               pd.nv * (16.0 * pd.nrep + 7);
+    }
+    int nv = 0;
+    for (int a = 0; a < 3; ++a) nv += (d.vmask >> a) & 1;
+    const double nb = d.nb;
+    return 8.0 * (8 * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;
+}
+
 double dev_last_kernel_ms(DeviceSolve* ds) { return ds ? ds->last_kernel_ms : 0.0; }
 
 void dev_get_rho(DeviceSolve* ds, double* out) {

@@ -203,6 +203,7 @@ API void SWESim_set_kernel_variant(SWESim* s, int v) {
         if (s->dev) hhg::dev_set_variant(s->dev, v);
     });
 }
+API double SWESim_flops_per_kstep(SWESim* s) { return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev) : 0.0; }
 API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev) : 0; }
 API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_last_kernel_ms(s->dev) : 0.0; }
 

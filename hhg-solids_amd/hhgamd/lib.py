@@ -53,6 +53,7 @@ def bind(lib):
         "SWESim_get_fields": ([V, _D, _D, _D], I), "SWESim_get_rho_k": ([V, _C], I),
         "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
         "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
+        "SWESim_flops_per_kstep": ([V], Dd),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -105,6 +106,9 @@ class Solve:
 
     def kernel(self):
         return self.lib.SWESim_get_kernel(self.ptr)
+
+    def flops_per_kstep(self):
+        return self.lib.SWESim_flops_per_kstep(self.ptr)
 
     def kernel_ms(self):
         return self.lib.SWESim_last_kernel_ms(self.ptr)

@@ -70,6 +70,9 @@ void SWESim_set_kernel_variant(SWESim* swe, int variant);
 /* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2,
  * 3 = element per lane; 0 before SWESim_init. */
 int SWESim_get_kernel(SWESim* swe);
+/* Arithmetic of the launched formulation per k-point and RK4 step (flops,
+ * a fused multiply-add counted as 2). */
+double SWESim_flops_per_kstep(SWESim* swe);
 /* Field tables exactly as the time loop consumes them: t[nt] (au),
  * E[3][nt], A[3][nt]. Returns 0 on success. */
 int SWESim_get_fields(SWESim* swe, double* t, double* E, double* A);

@@ -1,0 +1,60 @@
+"""Mixed precision (FP32 RK4 stages, FP64 state/initial state/current) against
+the FP64 path on the GPU.
+
+Tolerances (targets of the mixed path): J(t) relative L2 <= 1e-5 against FP64
+and every harmonic intensity within 1 % of FP64 up to the cutoff, where the
+cutoff is the highest odd harmonic within 1e-6 of the strongest one.
+"""
+import numpy as np
+import pytest
+
+from conftest import HALDANE, HMCASE0, make_param, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+J_TOL = 1e-5
+HARMONIC_TOL = 0.01
+
+
+def run(prm):
+    from hhgamd import lib
+    with lib.Solve(prm) as s:
+        s.run()
+        return s.current()
+
+
+def check(prm, lam=3000.0):
+    from hhgamd import spectrum
+    t, J64 = run(dict(prm, precision="fp64"))
+    _, J32 = run(dict(prm, precision="fp32"))
+    err = rel_l2(J32, J64)
+    worst = 0.0
+    for a in range(3):
+        if np.abs(J64[:, a]).max() == 0:
+            continue
+        q, I64 = spectrum.harmonic_intensities(t, J64[:, a], lam)
+        _, I32 = spectrum.harmonic_intensities(t, J32[:, a], lam)
+        qc = spectrum.cutoff_order(q, I64)
+        sel = q <= qc
+        worst = max(worst, float(np.max(np.abs(I32[sel] / I64[sel] - 1.0))))
+    print(f"J rel L2 fp32 vs fp64 = {err:.3e}, worst harmonic deviation up to cutoff = {worst:.3e}")
+    return err, worst
+
+
+def test_fp32_c2_haldane_200():
+    err, worst = check(make_param(HALDANE, nr=200))
+    assert err <= J_TOL and worst <= HARMONIC_TOL
+
+
+def test_fp32_hmcase0_readme():
+    err, worst = check(make_param(HMCASE0, nr=100))
+    assert err <= J_TOL and worst <= HARMONIC_TOL
+
+
+@pytest.mark.parametrize("nb", [4, 8])
+def test_fp32_synthetic(tmp_path, nb):
+    from hhgamd import tbgen
+    path = str(tmp_path / f"syn{nb}_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=11 + nb))
+    err, worst = check(make_param(path, nr=24, intensity=5e11, pol=(0.6, 0.8, 0.0)))
+    assert err <= J_TOL and worst <= HARMONIC_TOL
