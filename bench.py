@@ -91,6 +91,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size oracle J check")
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"],
+                    help="plain: the FP64 200x200 solve (default); batch: 64 mixed instances in one batched run; "
+                         "serve: Poisson request streams through the continuous-batching server")
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--rates", default="0.5,2,5,20", help="serve: arrival rates (requests/s) to sweep")
+    ap.add_argument("--requests", type=int, default=16, help="serve: requests per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
@@ -98,6 +104,11 @@ def main():
         sys.exit(2)
 
     from hhgamd import lib, tbgen
+
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
 
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
     path_tb = os.path.join(tmp, "haldane_tb.dat")
@@ -158,6 +169,76 @@ def main():
             import oracle
             _, Jref = oracle.run(path_tb, prm, mode="perk", nthreads=threads)
             out["j_rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(J - Jref) / np.linalg.norm(Jref))
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """64 seeded instances (pol angle, intensity, wavelength, nb in {2,4,8},
+    uneven 5-smooth k-grids) propagated by one batched run per step; the same
+    instances run one by one are timed for comparison."""
+    from hhgamd import lib, workloads
+    tmp = tempfile.mkdtemp(prefix="hhg_bench_")
+    params = workloads.c4(tmp, n=args.instances, precision=args.precision)
+    solves = [lib.Solve(p) for p in params]
+    total = sum(s.N * s.nt for s in solves)
+    batch = lib.Batch(solves)
+    for _ in range(args.warmup):
+        batch.run()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        batch.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+    wall = time.perf_counter() - t0
+    batch.close()
+    # the same instances one by one (one pass)
+    t1 = time.perf_counter()
+    for s in solves:
+        s.run()
+    seq = time.perf_counter() - t1
+    kinds = {}
+    for s, p in zip(solves, params):
+        kinds.setdefault(f"nb{s.nb}/k{s.kernel()}", 0)
+        kinds[f"nb{s.nb}/k{s.kernel()}"] += 1
+    for s in solves:
+        s.close()
+    out = {"metric": METRIC, "value": total * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+           "data": "synthetic",
+           "config": {"workload": f"batch-{len(params)}-mixed-nb248-{args.precision}", "instances": len(params),
+                      "kpoint_steps_per_pass": total, "kernels": kinds},
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "one_by_one_value": total / seq, "batch_speedup_vs_one_by_one": seq / (wall / args.steps)}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Poisson streams of mixed solves (nb in {2,4,8,16}, 5-smooth grids in
+    [48, 240], tmax U[60, 150]) through the server at each arrival rate;
+    reports p50/p99 latency (arrival to J available) and k-point*RK4-steps/s."""
+    from hhgamd import serve, workloads
+    tmp = tempfile.mkdtemp(prefix="hhg_bench_")
+    rates = [float(r) for r in args.rates.split(",")]
+    sweep = []
+    with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8) as srv:
+        warm = workloads.c5(tmp, 2, rate=100.0, seed=99, precision=args.precision)
+        serve.run_stream(srv, [(0.0, p) for _, p in warm])
+        for i, rate in enumerate(rates):
+            arrivals = workloads.c5(tmp, args.requests, rate=rate, seed=50 + i, precision=args.precision)
+            _, summ = serve.run_stream(srv, arrivals)
+            summ["rate_per_s"] = rate
+            sweep.append(summ)
+            print(json.dumps({"serve_rate": rate, **summ}), file=sys.stderr, flush=True)
+    best = max(sweep, key=lambda x: x["ksteps_per_s"])
+    out = {"metric": METRIC, "value": best["ksteps_per_s"], "unit": UNIT, "n_gpus": 1, "steps": args.requests,
+           "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+           "config": {"workload": f"serve-poisson-nb2-16-{args.precision}", "requests_per_rate": args.requests,
+                      "rates_per_s": rates},
+           "latency_ms": {"p50": best["p50_latency_s"] * 1e3, "p99": best["p99_latency_s"] * 1e3},
+           "sweep": sweep}
     print(json.dumps(out), flush=True)
 
 

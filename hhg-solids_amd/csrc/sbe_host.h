@@ -54,6 +54,12 @@ DeviceSolve* dev_create(const HostSolve& hs);
 // Propagates nt steps from the current device state. J_out: [nt][6].
 // Returns the device time in milliseconds.
 double dev_run(DeviceSolve* ds, int precision, double* J_out);
+// Asynchronous form: enqueue on the solve's stream, poll, then complete.
+void dev_launch(DeviceSolve* ds, int precision);
+bool dev_query(DeviceSolve* ds);
+double dev_complete(DeviceSolve* ds, double* J_out);
+// HBM footprint of a solve built from hs (bytes).
+size_t dev_bytes(const HostSolve& hs);
 void dev_get_rho(DeviceSolve* ds, double* rho_elem_major);
 // Device time of the propagation kernel alone in the last dev_run.
 double dev_last_kernel_ms(DeviceSolve* ds);
@@ -64,5 +70,15 @@ int dev_kernel_variant(DeviceSolve* ds);
 // Flops per k-point and RK4 step of the formulation the next run launches.
 double dev_flops_per_kstep(DeviceSolve* ds);
 void dev_destroy(DeviceSolve* ds);
+
+// Batched propagation: all added solves advance nt steps in one launch per
+// kernel class (different classes overlap on separate streams).
+struct Batch;
+Batch* batch_create();
+void batch_destroy(Batch* b);
+void batch_add(Batch* b, DeviceSolve* ds, int precision);
+int batch_size(Batch* b);
+// J_out[i]: [nt_i][6] for the i-th added solve. Returns device time (ms).
+double batch_run(Batch* b, const std::vector<double*>& J_out);
 
 }  // namespace hhg

@@ -27,10 +27,14 @@
 // path); the state rho and the current are always FP64.
 #include <hip/hip_runtime.h>
 
-#include <chrono>
+#include <algorithm>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <type_traits>
+#include <vector>
 
 #include "sbe_host.h"
 
@@ -121,19 +125,22 @@ struct JStage {
 // One lane per k. MAXU bounds the support size so that the per-k DFT phases
 // live in registers.
 template <int NB, int MAXU, typename T>
-__global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const double* __restrict__ blk,
-                                                     const double* __restrict__ steptab,
-                                                     const double* __restrict__ tw1,
-                                                     const double* __restrict__ tw2,
-                                                     const double* __restrict__ rho_in,
-                                                     double* __restrict__ rho_out, double* __restrict__ jpart,
-                                                     int nwaves, double dt) {
+__device__ __forceinline__ void thread_per_k_body(const SolveDesc& d, const InstPtrs& ip, int bid) {
+    const double* __restrict__ blk = ip.blk;
+    const double* __restrict__ steptab = ip.steptab;
+    const double* __restrict__ tw1 = ip.tw1;
+    const double* __restrict__ tw2 = ip.tw2;
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
     constexpr int E = NB * NB;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = bid * 64 + threadIdx.x;
     const bool active = k < d.N;
     const int kk = active ? k : 0;
     const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
-    const int wave = k >> 6;
+    const int wave = bid;
     const int lane = threadIdx.x & 63;
 
     cx<double> w0[MAXU];
@@ -288,16 +295,21 @@ __global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const do
 // The H coefficients of NREP pairs stay in registers for the whole time
 // loop; the rbc and v coefficients are read from LDS (broadcast reads).
 template <int NREP, int NX, int NV, typename T>
-__global__ __launch_bounds__(64) void k_pair2(const SolveDesc d, const PairDesc pd, const double* __restrict__ coef,
-                                              const double* __restrict__ steps, const double* __restrict__ tw1,
-                                              const double* __restrict__ tw2, const double* __restrict__ rho_in,
-                                              double* __restrict__ rho_out, double* __restrict__ jpart, int nwaves,
-                                              double dt) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
+    const double* __restrict__ coef = ip.pcoef;
+    const double* __restrict__ steps = ip.psteps;
+    const double* __restrict__ tw1 = ip.tw1;
+    const double* __restrict__ tw2 = ip.tw2;
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
+    const int k = bid * 64 + threadIdx.x;
     const bool active = k < d.N;
     const int kk = active ? k : 0;
     const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
-    const int wave = k >> 6;
+    const int wave = bid;
     const int lane = threadIdx.x & 63;
     constexpr int TCH = 16;
     using JS = JStage<3, TCH>;
@@ -458,24 +470,34 @@ __global__ __launch_bounds__(64) void k_pair2(const SolveDesc d, const PairDesc 
 // stage rho are staged through LDS and each lane forms its commutator
 // elements from row i / column j there. LDS per k: the support phases (ns
 // complex, FP64) and two nb*nb matrices of type T.
+__device__ __forceinline__ int lanes_per_k(int nb) {
+    int g = 1;
+    while (g < nb * nb && g < 64) g <<= 1;
+    return g;
+}
+
 template <int EPL, typename T>
-__global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const double* __restrict__ blk,
-                                                       const double* __restrict__ steptab,
-                                                       const double* __restrict__ tw1,
-                                                       const double* __restrict__ tw2,
-                                                       const double* __restrict__ rho_in,
-                                                       double* __restrict__ rho_out, double* __restrict__ jpart,
-                                                       int nwaves, double dt, int G) {
+__device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip, int bid) {
+    const double* __restrict__ blk = ip.blk;
+    const double* __restrict__ steptab = ip.steptab;
+    const double* __restrict__ tw1 = ip.tw1;
+    const double* __restrict__ tw2 = ip.tw2;
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
+    const int G = lanes_per_k(d.nb);
     extern __shared__ __align__(16) unsigned char smem[];
     const int NB = d.nb, E = NB * NB;
     const int kpb = blockDim.x / G;
     const int grp = threadIdx.x / G;
     const int l = threadIdx.x - grp * G;
-    const int k = blockIdx.x * kpb + grp;
+    const int k = bid * kpb + grp;
     const bool kvalid = k < d.N;
     const int kk = kvalid ? k : 0;
     const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int wave = (bid * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
 
     cx<double>* wS = reinterpret_cast<cx<double>*>(smem) + (size_t)grp * d.ns;
@@ -630,6 +652,40 @@ __global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const 
         }
 }
 
+// --------------------------------------------------------------------------
+// Launch wrappers. Single instance: descriptor in kernel arguments, block b
+// handles tile b. Batch: tiles[b] = (instance, tile of that instance), the
+// descriptors live in HBM (uniform, scalar loads).
+template <int NB, int MAXU, typename T>
+__global__ __launch_bounds__(64) void k_thread_per_k(const SolveDesc d, const InstPtrs ip) {
+    thread_per_k_body<NB, MAXU, T>(d, ip, blockIdx.x);
+}
+template <int NB, int MAXU, typename T>
+__global__ __launch_bounds__(64) void k_thread_per_k_batch(const DevInst* __restrict__ insts,
+                                                           const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    thread_per_k_body<NB, MAXU, T>(insts[t.x].d, insts[t.x].p, t.y);
+}
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_pair2(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
+    pair2_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
+}
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_pair2_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    pair2_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
+}
+template <int EPL, typename T>
+__global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const InstPtrs ip) {
+    elem_body<EPL, T>(d, ip, blockIdx.x);
+}
+template <int EPL, typename T>
+__global__ __launch_bounds__(256) void k_elem_per_lane_batch(const DevInst* __restrict__ insts,
+                                                             const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    elem_body<EPL, T>(insts[t.x].d, insts[t.x].p, t.y);
+}
+
 // J[ti][c] = (sum over waves, in wave order per lane stripe) / N. The
 // partials carry C components per wave: C = 6 (re/im of x, y, z) or C = 3
 // (real parts of the current along the axes ax0, ax1, ax2 -- the Hermitian
@@ -661,6 +717,34 @@ __global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpar
     }
 }
 
+// Batched reduction: block b reduces step rt[b].y of instance rt[b].x.
+__global__ __launch_bounds__(64) void k_reduce_j_batch(const DevInst* __restrict__ insts, const int2* __restrict__ rt) {
+    const int2 t = rt[blockIdx.x];
+    const DevInst& I = insts[t.x];
+    const int lane = threadIdx.x, C = I.C, nwaves = I.p.nwaves;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = lane; w < nwaves; w += 64) {
+        const double* p = I.p.jpart + ((size_t)t.y * nwaves + w) * C;
+        for (int c = 0; c < C; ++c) acc[c] += p[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
+    if (lane == 0) {
+        double* out = I.J + (size_t)t.y * 6;
+        const double N = (double)I.d.N;
+        if (C == 6) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = acc[c] / N;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (I.axis[c] >= 0) out[2 * I.axis[c]] = acc[c] / N;
+        }
+    }
+}
+
 // --------------------------------------------------------------------------
 struct DeviceSolve {
     SolveDesc d;
@@ -669,6 +753,8 @@ struct DeviceSolve {
     double *rho = nullptr, *rho_tmp = nullptr, *jpart = nullptr, *J = nullptr;
     PairDesc pd{};
     double *pcoef = nullptr, *psteps = nullptr;
+    double* hJ = nullptr;  // pinned host copy of J for asynchronous completion
+    int last_C = 6;
     size_t jpart_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
@@ -676,11 +762,20 @@ struct DeviceSolve {
     float last_kernel_ms = 0.f;
 };
 
+// Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the
+// solve's own stream): creating or destroying a solve never synchronises
+// other solves that run concurrently.
 template <typename V>
 static double* upload(const std::vector<V>& h, hipStream_t s) {
     double* p = nullptr;
-    HIP_CHECK(hipMalloc(&p, sizeof(V) * std::max<size_t>(h.size(), 1)));
+    HIP_CHECK(hipMallocAsync((void**)&p, sizeof(V) * std::max<size_t>(h.size(), 1), s));
     if (!h.empty()) HIP_CHECK(hipMemcpyAsync(p, h.data(), sizeof(V) * h.size(), hipMemcpyHostToDevice, s));
+    return p;
+}
+
+static double* dalloc(size_t bytes, hipStream_t s) {
+    double* p = nullptr;
+    HIP_CHECK(hipMallocAsync((void**)&p, std::max<size_t>(bytes, 16), s));
     return p;
 }
 
@@ -703,8 +798,9 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             ds->pcoef = upload(hs.pcoef, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
         }
-        HIP_CHECK(hipMalloc(&ds->rho_tmp, sizeof(double) * hs.rho0.size()));
-        HIP_CHECK(hipMalloc(&ds->J, sizeof(double) * 6 * (size_t)hs.d.nt));
+        ds->rho_tmp = dalloc(sizeof(double) * hs.rho0.size(), ds->stream);
+        ds->J = dalloc(sizeof(double) * 6 * (size_t)hs.d.nt, ds->stream);
+        HIP_CHECK(hipHostMalloc((void**)&ds->hJ, sizeof(double) * 6 * (size_t)hs.d.nt, hipHostMallocDefault));
         HIP_CHECK(hipStreamSynchronize(ds->stream));
     } catch (...) {
         dev_destroy(ds);
@@ -717,12 +813,25 @@ void dev_destroy(DeviceSolve* ds) {
     if (!ds) return;
     for (double* p : {ds->blk, ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, ds->J, ds->pcoef,
                       ds->psteps})
-        if (p) (void)hipFree(p);
+        if (p && ds->stream) (void)hipFreeAsync(p, ds->stream);
+    if (ds->stream) (void)hipStreamSynchronize(ds->stream);
+    if (ds->hJ) (void)hipHostFree(ds->hJ);
     if (ds->ev0) (void)hipEventDestroy(ds->ev0);
     if (ds->ev1) (void)hipEventDestroy(ds->ev1);
     if (ds->evk) (void)hipEventDestroy(ds->evk);
     if (ds->stream) (void)hipStreamDestroy(ds->stream);
     delete ds;
+}
+
+size_t dev_bytes(const HostSolve& hs) {
+    const SolveDesc& d = hs.d;
+    // propagation partials: at most 6 components per wave; elem tiles hold 256/G k in 4 waves
+    const int E = d.nb * d.nb;
+    int G = 1;
+    while (G < E && G < 64) G <<= 1;
+    const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
+    return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * hs.rho0.size() +
+                             hs.pcoef.size() + hs.psteps.size() + 6 * (size_t)d.nt + 6 * (size_t)d.nt * waves);
 }
 
 static int pow2ceil(int n) {
@@ -731,118 +840,331 @@ static int pow2ceil(int n) {
     return g;
 }
 
-static void ensure_jpart(DeviceSolve* ds, int nwaves, int C = 6) {
+// Which kernel and template instance a solve runs with.
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3 };
+struct LaunchKey {
+    int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
+    bool operator<(const LaunchKey& o) const {
+        return std::tie(kind, a, b, c, fp32) < std::tie(o.kind, o.a, o.b, o.c, o.fp32);
+    }
+};
+
+static LaunchKey classify(const DeviceSolve* ds, int precision) {
+    const SolveDesc& d = ds->d;
+    const PairDesc& pd = ds->pd;
+    LaunchKey k;
+    k.fp32 = precision != 0;
+    if (pd.nrep > 0 && pd.nrep <= 4 && ds->use_herm) {
+        k.kind = KIND_PAIR;
+        k.a = pd.nrep <= 3 ? 3 : 4;
+        if (pd.nx == 0) { k.b = 0; k.c = pd.nv <= 2 ? 2 : 3; }
+        else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
+        else { k.b = 3; k.c = 3; }
+        return k;
+    }
+    if (d.nb <= 2 && d.ns <= 32) {
+        k.kind = KIND_THREAD;
+        k.a = d.nb;
+        k.b = d.nb == 1 ? 16 : (d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32);
+        return k;
+    }
+    if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
+    k.kind = KIND_ELEM;
+    k.a = d.nb * d.nb <= 64 ? 1 : 4;
+    return k;
+}
+
+static int comps_of(const LaunchKey& k) { return k.kind == KIND_PAIR ? 3 : 6; }
+
+// Tiles (workgroups) of one instance and waves per tile.
+static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block) {
+    if (k.kind == KIND_ELEM) {
+        const int G = std::min(64, pow2ceil(d.nb * d.nb));
+        block = 256;
+        waves_per_tile = 4;
+        const int kpb = block / G;
+        return (d.N + kpb - 1) / kpb;
+    }
+    block = 64;
+    waves_per_tile = 1;
+    return (d.N + 63) / 64;
+}
+
+static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
+    if (k.kind != KIND_ELEM) return 0;
+    const int G = std::min(64, pow2ceil(d.nb * d.nb));
+    const int kpb = 256 / G;
+    const size_t esz = k.fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
+    return sizeof(cx<double>) * (size_t)kpb * d.ns + esz * (size_t)kpb * 2 * d.nb * d.nb;
+}
+
+// Dispatch over the compiled instances. F(kernel_single, kernel_batch) receives
+// the two kernel function pointers of the matching instance.
+template <typename F>
+static void dispatch(const LaunchKey& k, F&& f) {
+    auto go = [&](auto single64, auto batch64, auto single32, auto batch32) {
+        if (k.fp32) f(single32, batch32); else f(single64, batch64);
+    };
+    switch (k.kind) {
+        case KIND_PAIR:
+#define PAIR(R, X, V)                                                                                    \
+    if (k.a == R && k.b == X && k.c == V)                                                                \
+        return go(k_pair2<R, X, V, double>, k_pair2_batch<R, X, V, double>, k_pair2<R, X, V, float>,     \
+                  k_pair2_batch<R, X, V, float>);
+            PAIR(3, 0, 2) PAIR(4, 0, 2) PAIR(3, 0, 3) PAIR(4, 0, 3)
+            PAIR(3, 2, 2) PAIR(4, 2, 2) PAIR(3, 3, 3) PAIR(4, 3, 3)
+#undef PAIR
+            break;
+        case KIND_THREAD:
+#define THR(NB, U)                                                                                           \
+    if (k.a == NB && k.b == U)                                                                               \
+        return go(k_thread_per_k<NB, U, double>, k_thread_per_k_batch<NB, U, double>, k_thread_per_k<NB, U, float>, \
+                  k_thread_per_k_batch<NB, U, float>);
+            THR(1, 16) THR(2, 8) THR(2, 16) THR(2, 32)
+#undef THR
+            break;
+        case KIND_ELEM:
+            if (k.a == 1)
+                return go(k_elem_per_lane<1, double>, k_elem_per_lane_batch<1, double>, k_elem_per_lane<1, float>,
+                          k_elem_per_lane_batch<1, float>);
+            if (k.a == 4)
+                return go(k_elem_per_lane<4, double>, k_elem_per_lane_batch<4, double>, k_elem_per_lane<4, float>,
+                          k_elem_per_lane_batch<4, float>);
+            break;
+    }
+    throw std::runtime_error("no compiled kernel for this launch key");
+}
+
+static void ensure_jpart(DeviceSolve* ds, int nwaves, int C) {
     size_t need = sizeof(double) * C * (size_t)ds->d.nt * nwaves;
     if (need > ds->jpart_bytes) {
-        if (ds->jpart) HIP_CHECK(hipFree(ds->jpart));
+        if (ds->jpart) HIP_CHECK(hipFreeAsync(ds->jpart, ds->stream));
         ds->jpart = nullptr;
-        HIP_CHECK(hipMalloc(&ds->jpart, need));
+        ds->jpart = dalloc(need, ds->stream);
         ds->jpart_bytes = need;
     }
 }
 
-template <int EPL, typename T>
-static void launch_elems(DeviceSolve* ds, int& nwaves) {
-    const SolveDesc& d = ds->d;
-    const int E = d.nb * d.nb;
-    const int G = std::min(64, pow2ceil(E));
-    if ((E + G - 1) / G > EPL) throw std::runtime_error("k_elem_per_lane: too many elements per lane");
-    const int block = 256;
-    const int kpb = block / G;
-    const int nblk = (d.N + kpb - 1) / kpb;
-    nwaves = nblk * (block / 64);
-    ensure_jpart(ds, nwaves);
-    size_t lds = sizeof(cx<double>) * (size_t)kpb * d.ns + sizeof(cx<T>) * (size_t)kpb * 2 * E;
-    if (lds > 160 * 1024) throw std::runtime_error("k_elem_per_lane: LDS footprint exceeds 160 KiB");
-    if (lds > 64 * 1024)
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_elem_per_lane<EPL, T>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_elem_per_lane<EPL, T>), dim3(nblk), dim3(block), lds, ds->stream, d, ds->blk, ds->steptab,
-                       ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt, G);
+static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves) {
+    InstPtrs p;
+    p.blk = ds->blk; p.steptab = ds->steptab; p.tw1 = ds->tw1; p.tw2 = ds->tw2;
+    p.pcoef = ds->pcoef; p.psteps = ds->psteps; p.rho_in = ds->rho;
+    p.rho_out = ds->rho_tmp; p.jpart = ds->jpart; p.nwaves = nwaves; p.dt = ds->dt;
+    return p;
+}
+
+static void set_lds_limit(const void* fn, size_t lds) {
+    if (lds > 160 * 1024) throw std::runtime_error("LDS footprint exceeds 160 KiB");
+    if (lds > 64 * 1024) HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+// Launch the propagation kernel of one solve; returns the number of current
+// components per wave partial.
+static int launch_single(DeviceSolve* ds, int precision, int& nwaves) {
+    const LaunchKey k = classify(ds, precision);
+    int wpt = 0, block = 0;
+    const int ntiles = tiles_of(ds->d, k, wpt, block);
+    nwaves = ntiles * wpt;
+    const int C = comps_of(k);
+    ensure_jpart(ds, nwaves, C);
+    const InstPtrs ip = ptrs_of(ds, nwaves);
+    const size_t lds = lds_of(ds->d, k);
+    dispatch(k, [&](auto single, auto) {
+        set_lds_limit((const void*)single, lds);
+        if constexpr (std::is_invocable_v<decltype(single), SolveDesc, PairDesc, InstPtrs>)
+            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, ds->d, ds->pd, ip);
+        else
+            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, ds->d, ip);
+    });
     HIP_CHECK(hipGetLastError());
+    return C;
 }
 
-template <int NB, int MAXU, typename T>
-static void launch_threads(DeviceSolve* ds, int& nwaves) {
-    const SolveDesc& d = ds->d;
-    const int block = 64;
-    const int nblk = (d.N + block - 1) / block;
-    nwaves = nblk;
-    ensure_jpart(ds, nwaves);
-    hipLaunchKernelGGL((k_thread_per_k<NB, MAXU, T>), dim3(nblk), dim3(block), 0, ds->stream, d, ds->blk,
-                       ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt);
-    HIP_CHECK(hipGetLastError());
-}
-
-template <int NREP, int NX, int NV, typename T>
-static void launch_pair2(DeviceSolve* ds, int& nwaves) {
-    const SolveDesc& d = ds->d;
-    const int block = 64;
-    const int nblk = (d.N + block - 1) / block;
-    nwaves = nblk;
-    ensure_jpart(ds, nwaves, 3);
-    hipLaunchKernelGGL((k_pair2<NREP, NX, NV, T>), dim3(nblk), dim3(block), 0, ds->stream, d, ds->pd, ds->pcoef,
-                       ds->psteps, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, nwaves, ds->dt);
-    HIP_CHECK(hipGetLastError());
-}
-
-template <int NREP, typename T>
-static void pair2_nx_nv(DeviceSolve* ds, int& nwaves) {
-    const PairDesc& pd = ds->pd;
-    if (pd.nx == 0 && pd.nv <= 2) return launch_pair2<NREP, 0, 2, T>(ds, nwaves);
-    if (pd.nx == 0) return launch_pair2<NREP, 0, 3, T>(ds, nwaves);
-    if (pd.nx <= 2 && pd.nv <= 2) return launch_pair2<NREP, 2, 2, T>(ds, nwaves);
-    return launch_pair2<NREP, 3, 3, T>(ds, nwaves);
-}
-
-// Returns the number of current components per wave partial (6 or 3).
-template <typename T>
-static int launch_any(DeviceSolve* ds, int& nwaves) {
-    const SolveDesc& d = ds->d;
-    if (ds->pd.nrep > 0 && ds->use_herm) {
-        if (ds->pd.nrep <= 3) return pair2_nx_nv<3, T>(ds, nwaves), 3;
-        if (ds->pd.nrep <= 4) return pair2_nx_nv<4, T>(ds, nwaves), 3;
-        if (ds->pd.nrep <= 8) return pair2_nx_nv<8, T>(ds, nwaves), 3;
-        return pair2_nx_nv<PAIR_MAX, T>(ds, nwaves), 3;
-    }
-    if (d.nb == 1 && d.ns <= 16) return launch_threads<1, 16, T>(ds, nwaves), 6;
-    if (d.nb == 2 && d.ns <= 8) return launch_threads<2, 8, T>(ds, nwaves), 6;
-    if (d.nb == 2 && d.ns <= 16) return launch_threads<2, 16, T>(ds, nwaves), 6;
-    if (d.nb == 2 && d.ns <= 32) return launch_threads<2, 32, T>(ds, nwaves), 6;
-    if (d.nb * d.nb <= 64) return launch_elems<1, T>(ds, nwaves), 6;
-    if (d.nb <= 16) return launch_elems<4, T>(ds, nwaves), 6;
-    throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
-}
-
-double dev_run(DeviceSolve* ds, int precision, double* J_out) {
+void dev_launch(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
     int nwaves = 0;
     HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
-    const int C = precision == 0 ? launch_any<double>(ds, nwaves) : launch_any<float>(ds, nwaves);
+    const int C = launch_single(ds, precision, nwaves);
     HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
     int ax[3] = {-1, -1, -1};
     for (int v = 0; v < ds->pd.nv && C == 3; ++v) ax[v] = ds->pd.vaxis[v];
     hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, C, ax[0], ax[1], ax[2],
                        (double)d.N, ds->J);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(ds->hJ, ds->J, sizeof(double) * 6 * (size_t)d.nt, hipMemcpyDeviceToHost, ds->stream));
     HIP_CHECK(hipEventRecord(ds->ev1, ds->stream));
+}
+
+bool dev_query(DeviceSolve* ds) {
+    const hipError_t e = hipEventQuery(ds->ev1);
+    if (e == hipErrorNotReady) return false;
+    HIP_CHECK(e);
+    return true;
+}
+
+double dev_complete(DeviceSolve* ds, double* J_out) {
+    HIP_CHECK(hipEventSynchronize(ds->ev1));
     std::swap(ds->rho, ds->rho_tmp);
-    HIP_CHECK(hipMemcpyAsync(J_out, ds->J, sizeof(double) * 6 * (size_t)d.nt, hipMemcpyDeviceToHost, ds->stream));
-    HIP_CHECK(hipStreamSynchronize(ds->stream));
+    std::copy(ds->hJ, ds->hJ + 6 * (size_t)ds->d.nt, J_out);
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ds->ev0, ds->ev1));
     HIP_CHECK(hipEventElapsedTime(&ds->last_kernel_ms, ds->ev0, ds->evk));
     return ms;
 }
 
+double dev_run(DeviceSolve* ds, int precision, double* J_out) {
+    dev_launch(ds, precision);
+    return dev_complete(ds, J_out);
+}
+
+
+// --------------------------------------------------------------------------
+// Batched runs: every solve of the batch advances nt steps; solves that share
+// a launch key go into one launch whose tile table interleaves their
+// workgroups, longest solves first. Different keys run on separate streams
+// and overlap on the device.
+struct Batch {
+    std::vector<DeviceSolve*> items;
+    std::vector<int> prec;
+    std::vector<hipStream_t> streams;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<void*> scratch;  // device tables of the last run
+    float last_ms = 0.f;
+    ~Batch() {
+        for (void* p : scratch) (void)hipFree(p);
+        for (auto s : streams) (void)hipStreamDestroy(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+Batch* batch_create() {
+    Batch* b = new Batch();
+    try {
+        HIP_CHECK(hipEventCreate(&b->e0));
+        HIP_CHECK(hipEventCreate(&b->e1));
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+void batch_destroy(Batch* b) { delete b; }
+void batch_add(Batch* b, DeviceSolve* ds, int precision) {
+    for (auto* x : b->items)
+        if (x == ds) throw std::runtime_error("solve already in this batch");
+    b->items.push_back(ds);
+    b->prec.push_back(precision);
+}
+int batch_size(Batch* b) { return (int)b->items.size(); }
+
+double batch_run(Batch* b, const std::vector<double*>& J_out) {
+    if (b->items.empty()) return 0.0;
+    for (void* p : b->scratch) HIP_CHECK(hipFree(p));
+    b->scratch.clear();
+    std::map<LaunchKey, std::vector<int>> groups;
+    std::vector<LaunchKey> keys(b->items.size());
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        keys[i] = classify(b->items[i], b->prec[i]);
+        groups[keys[i]].push_back((int)i);
+    }
+    while (b->streams.size() < groups.size()) {
+        hipStream_t s;
+        HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->streams.push_back(s);
+    }
+    // finish any pending single-solve work before the batch reads the states
+    for (auto* ds : b->items) HIP_CHECK(hipStreamSynchronize(ds->stream));
+    HIP_CHECK(hipDeviceSynchronize());
+    HIP_CHECK(hipEventRecord(b->e0, b->streams[0]));
+    for (size_t g = 1; g < groups.size(); ++g) HIP_CHECK(hipStreamWaitEvent(b->streams[g], b->e0, 0));
+    auto devalloc = [&](size_t bytes) {
+        void* p = nullptr;
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        b->scratch.push_back(p);
+        return p;
+    };
+    size_t gi = 0;
+    for (auto& [key, idx] : groups) {
+        hipStream_t st = b->streams[gi++];
+        std::vector<DevInst> insts(idx.size());
+        std::vector<int2> tiles, rt;
+        size_t lds = 0;
+        int block = 0;
+        // longest solves (most tiles x steps) first
+        std::vector<int> order(idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) order[j] = (int)j;
+        std::vector<double> cost(idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) cost[j] = (double)b->items[idx[j]]->d.nt;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+        std::vector<int> ntiles(idx.size());
+        for (size_t j = 0; j < idx.size(); ++j) {
+            DeviceSolve* ds = b->items[idx[j]];
+            int wpt = 0;
+            ntiles[j] = tiles_of(ds->d, key, wpt, block);
+            const int nwaves = ntiles[j] * wpt;
+            ensure_jpart(ds, nwaves, comps_of(key));
+            DevInst& I = insts[j];
+            I.d = ds->d;
+            I.pd = ds->pd;
+            I.p = ptrs_of(ds, nwaves);
+            I.J = ds->J;
+            I.C = comps_of(key);
+            for (int c = 0; c < 3; ++c) I.axis[c] = (I.C == 3 && c < ds->pd.nv) ? ds->pd.vaxis[c] : -1;
+            lds = std::max(lds, lds_of(ds->d, key));
+            for (int t = 0; t < ds->d.nt; ++t) rt.push_back(make_int2((int)j, t));
+        }
+        // interleave the tiles of equally long solves round-robin, longest first
+        std::vector<int> next(idx.size(), 0);
+        for (size_t pos = 0; pos < order.size();) {
+            size_t end = pos;
+            while (end < order.size() && cost[order[end]] == cost[order[pos]]) ++end;
+            bool any = true;
+            while (any) {
+                any = false;
+                for (size_t q = pos; q < end; ++q) {
+                    const int j = order[q];
+                    if (next[j] < ntiles[j]) { tiles.push_back(make_int2(j, next[j]++)); any = true; }
+                }
+            }
+            pos = end;
+        }
+        DevInst* d_insts = (DevInst*)devalloc(sizeof(DevInst) * insts.size());
+        int2* d_tiles = (int2*)devalloc(sizeof(int2) * tiles.size());
+        int2* d_rt = (int2*)devalloc(sizeof(int2) * rt.size());
+        HIP_CHECK(hipMemcpyAsync(d_insts, insts.data(), sizeof(DevInst) * insts.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(d_rt, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice, st));
+        dispatch(key, [&](auto, auto batch) {
+            set_lds_limit((const void*)batch, lds);
+            hipLaunchKernelGGL(batch, dim3((unsigned)tiles.size()), dim3(block), lds, st, d_insts, d_tiles);
+        });
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_j_batch, dim3((unsigned)rt.size()), dim3(64), 0, st, d_insts, d_rt);
+        HIP_CHECK(hipGetLastError());
+    }
+    for (size_t g = 1; g < groups.size(); ++g) {
+        hipEvent_t ev;
+        HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIP_CHECK(hipEventRecord(ev, b->streams[g]));
+        HIP_CHECK(hipStreamWaitEvent(b->streams[0], ev, 0));
+        HIP_CHECK(hipEventDestroy(ev));
+    }
+    HIP_CHECK(hipEventRecord(b->e1, b->streams[0]));
+    for (size_t i = 0; i < b->items.size(); ++i) {
+        DeviceSolve* ds = b->items[i];
+        std::swap(ds->rho, ds->rho_tmp);
+        HIP_CHECK(hipMemcpyAsync(J_out[i], ds->J, sizeof(double) * 6 * (size_t)ds->d.nt, hipMemcpyDeviceToHost,
+                                 b->streams[0]));
+    }
+    HIP_CHECK(hipStreamSynchronize(b->streams[0]));
+    HIP_CHECK(hipEventElapsedTime(&b->last_ms, b->e0, b->e1));
+    return b->last_ms;
+}
+
 void dev_set_variant(DeviceSolve* ds, int variant) { ds->use_herm = (variant == 0); }
 
-int dev_kernel_variant(DeviceSolve* ds) {
-    const SolveDesc& d = ds->d;
-    if (ds->pd.nrep > 0 && ds->use_herm) return 2;                    // k_pair2
-    if (d.nb <= 2 && d.ns <= 32) return 1;                           // k_thread_per_k
-    return 3;                                                         // k_elem_per_lane
-}
+int dev_kernel_variant(DeviceSolve* ds) { return classify(ds, 0).kind; }
 
 // Arithmetic of the launched formulation per k-point and RK4 step, counting
 // a fused multiply-add as 2 flops. Hermitian nb = 2 kernel: pair phases

@@ -61,4 +61,22 @@ struct PairDesc {
     int n1[PAIR_MAX], n2[PAIR_MAX];
 };
 
+// Device pointers and scalars of one instance (solve) on the GPU.
+struct InstPtrs {
+    const double *blk, *steptab, *tw1, *tw2, *pcoef, *psteps, *rho_in;
+    double *rho_out, *jpart;
+    int nwaves;          // waves (current partials per step) of this instance's launch
+    double dt;
+};
+
+// Descriptor of one instance of a batched launch, resident in HBM.
+struct DevInst {
+    SolveDesc d;
+    PairDesc pd;
+    InstPtrs p;
+    double* J;           // [nt][6] reduced current
+    int C;               // current components per wave partial (6, or 3 on the Hermitian path)
+    int axis[3];         // axes of the 3 components when C == 3 (-1 = unused)
+};
+
 }  // namespace hhg

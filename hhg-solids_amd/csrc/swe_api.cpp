@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "sbe_host.h"
+#include "sbe_server.h"
 
 using hhg::cplx;
 
@@ -66,6 +67,16 @@ static void guarded(SWESim* s, F&& f) {
     }
 }
 
+static hhg::SolveParams params_of(const Settings& st) {
+    hhg::SolveParams p;
+    p.nr1 = st.nr1; p.nr2 = st.nr2; p.nr3 = st.nr3;
+    p.tmax = st.tmax; p.dt = st.dt;
+    p.lambda_nm = st.lambda_nm; p.tmax_field = st.tmax_field; p.intensity = st.intensity_wcm2;
+    p.s[0] = st.sx; p.s[1] = st.sy; p.s[2] = st.sz;
+    p.phi[0] = st.phi_x; p.phi[1] = st.phi_y; p.phi[2] = st.phi_z;
+    return p;
+}
+
 static void sim_init(SWESim* s, bool device = true) {
     if (!s->settings) throw std::runtime_error("SWESim_init: no Settings (call SWESim_set_settings)");
     Settings& st = *s->settings;
@@ -74,12 +85,7 @@ static void sim_init(SWESim* s, bool device = true) {
     st.path_tb = s->path_tb;
     st.num_orb = m.nb;
     st.num_sites = m.nR;
-    hhg::SolveParams p;
-    p.nr1 = st.nr1; p.nr2 = st.nr2; p.nr3 = st.nr3;
-    p.tmax = st.tmax; p.dt = st.dt;
-    p.lambda_nm = st.lambda_nm; p.tmax_field = st.tmax_field; p.intensity = st.intensity_wcm2;
-    p.s[0] = st.sx; p.s[1] = st.sy; p.s[2] = st.sz;
-    p.phi[0] = st.phi_x; p.phi[1] = st.phi_y; p.phi[2] = st.phi_z;
+    const hhg::SolveParams p = params_of(st);
     auto hs = std::make_unique<hhg::HostSolve>(hhg::build_solve(m, p));
     hhg::DeviceSolve* dev = device ? hhg::dev_create(*hs) : nullptr;
     if (dev) hhg::dev_set_variant(dev, s->variant);
@@ -206,5 +212,164 @@ API void SWESim_set_kernel_variant(SWESim* s, int v) {
 API double SWESim_flops_per_kstep(SWESim* s) { return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev) : 0.0; }
 API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev) : 0; }
 API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_last_kernel_ms(s->dev) : 0.0; }
+
+}  // extern "C"
+
+// ---- batches -----------------------------------------------------------
+struct hhg_batch {
+    hhg::Batch* b = nullptr;
+    std::vector<SWESim*> sims;
+    std::string err;
+    ~hhg_batch() { hhg::batch_destroy(b); }
+};
+
+template <typename F>
+static int bguard(hhg_batch* b, F&& f) {
+    try {
+        if (!b) throw std::runtime_error("null batch handle");
+        f();
+        b->err.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        if (b) b->err = e.what();
+        return 1;
+    }
+}
+
+extern "C" {
+
+API hhg_batch* hhg_batch_new() {
+    try {
+        auto* b = new hhg_batch();
+        b->b = hhg::batch_create();
+        return b;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+API void hhg_batch_delete(hhg_batch* b) { delete b; }
+API int hhg_batch_add(hhg_batch* b, SWESim* s) {
+    return bguard(b, [&] {
+        if (!s || !s->dev) throw std::runtime_error("hhg_batch_add: solve not initialised on the device");
+        hhg::batch_add(b->b, s->dev, s->precision);
+        b->sims.push_back(s);
+    });
+}
+API int hhg_batch_size(hhg_batch* b) { return b ? (int)b->sims.size() : 0; }
+API int hhg_batch_run(hhg_batch* b) {
+    return bguard(b, [&] {
+        std::vector<std::vector<double>> J(b->sims.size());
+        std::vector<double*> out(b->sims.size());
+        for (size_t i = 0; i < b->sims.size(); ++i) {
+            J[i].resize((size_t)6 * b->sims[i]->host->d.nt);
+            out[i] = J[i].data();
+        }
+        const double ms = hhg::batch_run(b->b, out);
+        for (size_t i = 0; i < b->sims.size(); ++i) {
+            SWESim* s = b->sims[i];
+            const int nt = s->host->d.nt;
+            for (int a = 0; a < 3; ++a)
+                for (int ti = 0; ti < nt; ++ti)
+                    s->J[a].push_back(cplx(J[i][(size_t)6 * ti + 2 * a], J[i][(size_t)6 * ti + 2 * a + 1]));
+            s->last_ms = ms;
+        }
+    });
+}
+}  // extern "C"
+
+// ---- server ------------------------------------------------------------
+struct hhg_server {
+    std::unique_ptr<hhg::Server> srv;
+};
+
+template <typename F>
+static int sguard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+extern "C" {
+
+API hhg_server* hhg_server_new(double hbm_budget_gb, int max_inflight, int host_threads) {
+    try {
+        hhg::ServerConfig c;
+        if (hbm_budget_gb > 0) c.hbm_budget = (size_t)(hbm_budget_gb * (double)(1ull << 30));
+        if (max_inflight > 0) c.max_inflight = max_inflight;
+        if (host_threads > 0) c.host_threads = host_threads;
+        auto* s = new hhg_server();
+        s->srv = std::make_unique<hhg::Server>(c);
+        return s;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+API void hhg_server_delete(hhg_server* s) { delete s; }
+API long long hhg_server_submit(hhg_server* s, const char* path_tb, Settings* st, int precision) {
+    long long id = -1;
+    sguard([&] {
+        if (!s || !st || !path_tb) throw std::runtime_error("hhg_server_submit: null argument");
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 or 1");
+        id = s->srv->submit(path_tb, params_of(*st), precision);
+    });
+    return id;
+}
+API int hhg_server_wait(hhg_server* s, long long id, double timeout_s) {
+    int st = -1;
+    if (sguard([&] { st = s->srv->wait(id, timeout_s); })) return -1;
+    return st;
+}
+API int hhg_server_state(hhg_server* s, long long id) {
+    int st = -1;
+    if (sguard([&] { st = s->srv->state(id); })) return -1;
+    return st;
+}
+API int hhg_server_nt(hhg_server* s, long long id) {
+    int n = -1;
+    if (sguard([&] { n = s->srv->nt(id); })) return -1;
+    return n;
+}
+API int hhg_server_result(hhg_server* s, long long id, double* t, cplx* jx, cplx* jy, cplx* jz) {
+    return sguard([&] {
+        const int nt = s->srv->nt(id);
+        std::vector<double> J((size_t)6 * nt);
+        s->srv->result(id, t, J.data());
+        cplx* out[3] = {jx, jy, jz};
+        for (int a = 0; a < 3; ++a)
+            if (out[a])
+                for (int ti = 0; ti < nt; ++ti) out[a][ti] = cplx(J[(size_t)6 * ti + 2 * a], J[(size_t)6 * ti + 2 * a + 1]);
+    });
+}
+API int hhg_server_timing(hhg_server* s, long long id, double* out7) {
+    return sguard([&] { s->srv->timing(id, out7); });
+}
+API int hhg_server_release(hhg_server* s, long long id) {
+    return sguard([&] { s->srv->release(id); });
+}
+API int hhg_server_stats(hhg_server* s, double* out) {
+    return sguard([&] {
+        const hhg::ServerStats st = s->srv->stats();
+        const double v[10] = {(double)st.submitted, (double)st.completed, (double)st.failed, (double)st.inflight,
+                              (double)st.queued, (double)st.peak_inflight, st.ksteps, st.bytes_in_use,
+                              st.peak_bytes, 0.0};
+        std::copy(v, v + 10, out);
+    });
+}
+API const char* hhg_server_error(hhg_server* s, long long id) {
+    static thread_local std::string msg;
+    msg.clear();
+    if (sguard([&] { msg = s->srv->error(id); })) msg = g_last_error;
+    return msg.c_str();
+}
+
+API double hhg_batch_last_ms(hhg_batch* b) { return (b && !b->sims.empty()) ? b->sims[0]->last_ms : 0.0; }
+API const char* hhg_batch_get_error(hhg_batch* b) { return b ? b->err.c_str() : "null batch handle"; }
 
 }  // extern "C"

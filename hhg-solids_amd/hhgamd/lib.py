@@ -54,6 +54,9 @@ def bind(lib):
         "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
         "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
         "SWESim_flops_per_kstep": ([V], Dd),
+        "hhg_batch_new": ([], V), "hhg_batch_delete": ([V], None), "hhg_batch_add": ([V, V], I),
+        "hhg_batch_size": ([V], I), "hhg_batch_run": ([V], I), "hhg_batch_last_ms": ([V], Dd),
+        "hhg_batch_get_error": ([V], P),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -139,6 +142,38 @@ class Solve:
         if self.st:
             self.lib.Settings_delete(self.st)
             self.st = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many solves propagated together (hhg_batch_* of include/hhg_swe.h)."""
+
+    def __init__(self, solves, lib=None):
+        self.lib = bind(lib or load())
+        self.solves = list(solves)
+        self.ptr = self.lib.hhg_batch_new()
+        if not self.ptr:
+            raise RuntimeError("hhg_batch_new: " + self.lib.hhg_last_error().decode())
+        for s in self.solves:
+            self._check(self.lib.hhg_batch_add(self.ptr, s.ptr), "add")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"hhg_batch_{what}: {self.lib.hhg_batch_get_error(self.ptr).decode()}")
+
+    def run(self):
+        self._check(self.lib.hhg_batch_run(self.ptr), "run")
+        return self.lib.hhg_batch_last_ms(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            self.lib.hhg_batch_delete(self.ptr)
+            self.ptr = None
 
     def __enter__(self):
         return self

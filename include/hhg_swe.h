@@ -85,6 +85,43 @@ double SWESim_last_run_ms(SWESim* swe);
 /* Device time of the propagation kernel alone in the last run (ms). */
 double SWESim_last_kernel_ms(SWESim* swe);
 
+/* ---- batches (no counterpart in the reference) -------------------------
+ * A batch propagates many initialised solves together: solves that use the
+ * same kernel share one launch whose workgroups interleave their k-tiles;
+ * different kernels run concurrently. Each member's current is appended as
+ * by SWESim_run_simulation. Functions return 0 on success. */
+typedef struct hhg_batch hhg_batch;
+hhg_batch* hhg_batch_new(void);
+void hhg_batch_delete(hhg_batch* b);
+int hhg_batch_add(hhg_batch* b, SWESim* swe);
+int hhg_batch_size(hhg_batch* b);
+int hhg_batch_run(hhg_batch* b);
+double hhg_batch_last_ms(hhg_batch* b);
+const char* hhg_batch_get_error(hhg_batch* b);
+
+/* ---- solve server (no counterpart in the reference) ----------------------
+ * Continuous batching: requests are prepared on host threads, admitted in
+ * arrival order when their HBM footprint fits the budget and one of
+ * max_inflight slots is free, and run concurrently on separate HIP streams.
+ * The Settings block is copied at submit. Request states: 0 queued,
+ * 1 preparing, 2 prepared, 3 running, 4 done, 5 failed. */
+typedef struct hhg_server hhg_server;
+hhg_server* hhg_server_new(double hbm_budget_gb, int max_inflight, int host_threads);
+void hhg_server_delete(hhg_server* s);  /* completes all submitted work first */
+long long hhg_server_submit(hhg_server* s, const char* path_tb, Settings* settings, int precision); /* -1: error */
+int hhg_server_wait(hhg_server* s, long long id, double timeout_s);  /* state; timeout < 0 waits */
+int hhg_server_state(hhg_server* s, long long id);
+int hhg_server_nt(hhg_server* s, long long id);
+int hhg_server_result(hhg_server* s, long long id, double* t, hhg_cdouble* jx, hhg_cdouble* jy,
+                      hhg_cdouble* jz);
+/* out7: submit, prepared, admitted, done (steady-clock seconds), device ms, k-point steps, bytes */
+int hhg_server_timing(hhg_server* s, long long id, double* out7);
+int hhg_server_release(hhg_server* s, long long id);
+/* out10: submitted, completed, failed, inflight, queued, peak inflight, k-point steps, bytes in use,
+ * peak bytes, reserved */
+int hhg_server_stats(hhg_server* s, double* out10);
+const char* hhg_server_error(hhg_server* s, long long id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,68 @@
+"""Batched propagation (hhg_batch_*): each member of a mixed batch gives the
+same current as its solve run alone, and the oracle's current."""
+import numpy as np
+import pytest
+
+from conftest import HALDANE, HMCASE0, make_param, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SAME_TOL = 1e-13     # batch member vs the same solve alone (identical arithmetic)
+FP64_TOL = 1e-10     # vs the oracle
+
+
+def _mixed_params(tmp_path):
+    from hhgamd import tbgen
+    files = {}
+    for nb in (2, 3, 4, 8):
+        files[nb] = str(tmp_path / f"syn{nb}_tb.dat")
+        tbgen.write_tb(files[nb], *tbgen.synthetic(nb, seed=20 + nb))
+    short = dict(tmax=8.0, tmax_field=7.0, intensity=2e12)
+    return [
+        make_param(HALDANE, nr=(16, 20), pol=(0.6, 0.8, 0.0), **short),
+        make_param(HMCASE0, nr=(12, 12), pol=(0.0, 1.0, 0.0), **short),
+        make_param(HALDANE, nr=(10, 14), pol=(1.0, 0.0, 0.0), tmax=5.0, tmax_field=4.0, intensity=1e12),
+        make_param(files[2], nr=(8, 10), pol=(0.6, 0.8, 0.0), **short),
+        make_param(files[3], nr=(6, 8), pol=(0.6, 0.8, 0.0), **short),
+        make_param(files[4], nr=(8, 8), pol=(0.6, 0.8, 0.0), **short),
+        make_param(files[8], nr=(6, 6), pol=(0.8, 0.6, 0.0), tmax=4.0, tmax_field=3.0, intensity=2e12),
+        dict(make_param(files[4], nr=(6, 8), pol=(0.6, 0.8, 0.0), **short), precision="fp32"),
+    ]
+
+
+def test_batch_members_equal_single_runs(tmp_path):
+    from hhgamd import lib
+    params = _mixed_params(tmp_path)
+    alone = []
+    for p in params:
+        with lib.Solve(p) as s:
+            s.run()
+            alone.append(s.current()[1])
+    solves = [lib.Solve(p) for p in params]
+    kinds = sorted({s.kernel() for s in solves})
+    assert kinds == [1, 2, 3]  # the batch mixes all three kernels
+    with lib.Batch(solves) as b:
+        b.run()
+    for p, s, Ja in zip(params, solves, alone):
+        Jb = s.current()[1]
+        err = rel_l2(Jb, Ja)
+        print(f"nr={p['nr1']}x{p['nr2']} kernel={s.kernel()} batch vs alone rel L2 = {err:.1e} "
+              f"bitwise={np.array_equal(Jb, Ja)}")
+        assert err <= SAME_TOL
+        s.close()
+
+
+def test_batch_vs_oracle_and_continuation(tmp_path, oracle_lib):
+    from hhgamd import lib
+    params = _mixed_params(tmp_path)[:3]
+    solves = [lib.Solve(p) for p in params]
+    with lib.Batch(solves) as b:
+        b.run()
+        rho1 = [s.rho_k() for s in solves]
+        b.run()  # continues from the evolved states; the first nt samples stay
+    for p, s, r1 in zip(params, solves, rho1):
+        _, J = s.current()
+        _, Jref = oracle_lib.run(p["path_tb"], p, mode="perk")
+        assert rel_l2(J, Jref) <= FP64_TOL
+        assert np.abs(s.rho_k() - r1).max() > 0
+        s.close()

@@ -127,3 +127,21 @@ def test_pyswe_mirror_settings_only(native):
     with pytest.raises(RuntimeError, match="cannot open"):
         SWE(Settings(bad))
     st.delete()
+
+
+def test_server_failure_paths_without_gpu(native, tmp_path):
+    """Preparation failures (missing file, footprint over the HBM budget, bad
+    grid) are reported per request and never reach the device."""
+    from hhgamd import serve
+    with serve.Server(hbm_budget_gb=0.001, max_inflight=2, host_threads=2) as srv:
+        a = srv.submit(make_param(str(tmp_path / "nope_tb.dat"), nr=8, tmax=1.0))
+        b = srv.submit(make_param(HALDANE, nr=200))
+        c = srv.submit(make_param(HALDANE, nr=(7, 8), tmax=1.0))
+        assert srv.wait(a) == "failed" and "cannot open" in srv.error(a)
+        assert srv.wait(b) == "failed" and "budget" in srv.error(b)
+        assert srv.wait(c) == "failed" and "even" in srv.error(c)
+        st = srv.stats()
+        assert st["failed"] == 3 and st["completed"] == 0 and st["peak_inflight"] == 0
+    with pytest.raises(RuntimeError, match="precision"):
+        with serve.Server() as srv:
+            srv.submit(dict(make_param(HALDANE, nr=8, tmax=1.0), precision="fp8"))
