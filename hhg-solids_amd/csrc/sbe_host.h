@@ -43,7 +43,8 @@ struct HostSolve {
     std::vector<double> t, E[3], A[3];
     // Hermitian nb = 2 pack (valid when pd.nrep > 0)
     PairDesc pd{};
-    std::vector<double> pcoef;       // PAIR_HEAD + PAIR_MAX*PAIR_REP
+    std::vector<double> pcoef;       // PAIR_HEAD + PAIR_MAX*PAIR_REP (when pd.vpaired)
+    std::vector<double> scoef;       // PS_HEAD + PAIR_MAX*PS_REP
     std::vector<double> psteps;      // nt * pd.stride
 };
 HostSolve build_solve(const TBModel& m, const SolveParams& p);

@@ -48,6 +48,17 @@ namespace hhg {
                                      " at " __FILE__ ":" + std::to_string(__LINE__));       \
     } while (0)
 
+// Read-only tables with wave-uniform addresses are read through the constant
+// address space, which lets the compiler use scalar (SMEM) loads.
+typedef const __attribute__((address_space(4))) double* cdptr;
+__device__ __forceinline__ cdptr as_const(const double* p) { return (cdptr)(p); }
+// An opaque zero: offsets a per-step load so it is not hoisted out of the loop.
+__device__ __forceinline__ int opaque_zero() {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
+}
+
 template <typename T>
 struct cx {
     T re, im;
@@ -72,6 +83,10 @@ __device__ __forceinline__ cx<T> cast(cx<S> a) {
 }
 template <typename T>
 __device__ __forceinline__ cx<T> ld(const double* p) {  // complex from interleaved doubles
+    return {(T)p[0], (T)p[1]};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld(cdptr p) {
     return {(T)p[0], (T)p[1]};
 }
 
@@ -296,8 +311,8 @@ __device__ __forceinline__ void thread_per_k_body(const SolveDesc& d, const Inst
 // loop; the rbc and v coefficients are read from LDS (broadcast reads).
 template <int NREP, int NX, int NV, typename T>
 __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
-    const double* __restrict__ coef = ip.pcoef;
-    const double* __restrict__ steps = ip.psteps;
+    const cdptr coef = as_const(ip.pcoef);
+    const cdptr steps = as_const(ip.psteps);
     const double* __restrict__ tw1 = ip.tw1;
     const double* __restrict__ tw2 = ip.tw2;
     const double* __restrict__ rho_in = ip.rho_in;
@@ -327,7 +342,7 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     for (int r = 0; r < NREP; ++r) {
         const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
         w0[r] = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
-        const double* cr = coef + PAIR_HEAD + PAIR_REP * r;
+        const cdptr cr = coef + PAIR_HEAD + PAIR_REP * r;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             hc[r][j] = (T)(cr[j] + lz);
@@ -362,7 +377,7 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     // per-step row, loaded one step ahead
     double ph[NREP][2], ef[3][NXA];
     auto load_row = [&](int ti, double (&a)[NREP][2], double (&e)[3][NXA]) {
-        const double* row = steps + (size_t)ti * pd.stride;
+        const cdptr row = steps + (size_t)ti * pd.stride;
 #pragma unroll
         for (int r = 0; r < NREP; ++r) { a[r][0] = row[PAIR_STEP_HEAD + 2 * r]; a[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
 #pragma unroll
@@ -452,6 +467,143 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int x = 0; x < NXA; ++x) ef[j][x] = efn[j][x];
+    }
+    if (active) {
+        double* o = rho_out + 2 * (size_t)k;
+        const size_t st = 2 * (size_t)d.N;
+        o[0] = p; o[1] = 0.0;
+        o[st] = c.re; o[st + 1] = c.im;
+        o[2 * st] = c.re; o[2 * st + 1] = -c.im;
+        o[3 * st] = q; o[3 * st + 1] = 0.0;
+    }
+}
+
+// --------------------------------------------------------------------------
+// nb = 2, Hermitian H and rbc, current possibly complex: pairs streamed.
+// The phases of up to NREP pairs stay in VGPRs; the coefficients (PS_REP per
+// pair) arrive through scalar loads every step -- the coefficient pointer is
+// laundered per step so that the loads are not hoisted out of the time loop
+// into more SGPRs than exist. Unused pair slots have phase 0.
+template <int NREP, int NX, int NV, typename T>
+__device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
+    const cdptr steps = as_const(ip.psteps);
+    const double* __restrict__ tw1 = ip.tw1;
+    const double* __restrict__ tw2 = ip.tw2;
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
+    const int k = bid * 64 + threadIdx.x;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = bid;
+    const int lane = threadIdx.x & 63;
+    constexpr int C = 2 * NV;
+    constexpr int TCH = 64 / C;
+    using JS = JStage<C, TCH>;
+    __shared__ double jl[64 * JS::ROW];
+    constexpr int NXA = NX > 0 ? NX : 1;
+
+    cx<double> w0[NREP];
+#pragma unroll
+    for (int r = 0; r < NREP; ++r) {
+        w0[r] = {0.0, 0.0};
+        if (r < pd.nrep) {
+            const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
+            w0[r] = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
+        }
+    }
+    double p = 0, q = 0;
+    cx<double> c = {0, 0};
+    if (active) {
+        p = rho_in[0 * 2 * (size_t)d.N + 2 * k];
+        c = ld<double>(rho_in + 1 * 2 * (size_t)d.N + 2 * k);
+        q = rho_in[3 * 2 * (size_t)d.N + 2 * k];
+    }
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    for (int ti = 0; ti < d.nt; ++ti) {
+        const cdptr row = steps + (size_t)ti * pd.stride;
+        const cdptr cf = as_const(ip.scoef) + opaque_zero();
+        cx<double> w[NREP];
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) w[r] = cmul(w0[r], ld<double>(row + PAIR_STEP_HEAD + 2 * r));
+        T hs[4], xs[NXA][4];
+        double mr[NV][4], mi[NV][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            hs[j] = (T)cf[j];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) xs[x][j] = (T)cf[4 + 4 * x + j];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { mr[v][j] = cf[16 + 8 * v + j]; mi[v][j] = cf[16 + 8 * v + 4 + j]; }
+        }
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const cdptr cr = cf + PS_HEAD + PS_REP * r;
+            const T wr = (T)w[r].re, wi = (T)w[r].im;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hs[j] = fmaT((T)cr[2 * j + 1], wi, fmaT((T)cr[2 * j], wr, hs[j]));
+#pragma unroll
+            for (int x = 0; x < NX; ++x)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    xs[x][j] = fmaT((T)cr[8 + 8 * x + 2 * j + 1], wi, fmaT((T)cr[8 + 8 * x + 2 * j], wr, xs[x][j]));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const cdptr cv = cr + 32 + 16 * v;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    mr[v][j] = fmaT(cv[2 * j + 1], w[r].im, fmaT(cv[2 * j], w[r].re, mr[v][j]));
+                    mi[v][j] = fmaT(cv[8 + 2 * j + 1], w[r].im, fmaT(cv[8 + 2 * j], w[r].re, mi[v][j]));
+                }
+            }
+        }
+        T A, D, Br, Bi;
+        auto stage_h = [&](int j) {
+            A = hs[0]; D = hs[1]; Br = hs[2]; Bi = hs[3];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const T f = (T)row[3 * j + x];
+                A = fmaT(f, xs[x][0], A); D = fmaT(f, xs[x][1], D);
+                Br = fmaT(f, xs[x][2], Br); Bi = fmaT(f, xs[x][3], Bi);
+            }
+        };
+        T kp, kr, ki;
+        auto deriv = [&](T sp, T sq, T scr, T sci) {
+            kp = (T)2 * fmaT(Bi, scr, -(Br * sci));
+            const T ad = A - D, qp = sq - sp;
+            kr = fmaT(ad, sci, qp * Bi);
+            ki = -fmaT(ad, scr, qp * Br);
+        };
+        const T P = (T)p, Q = (T)q, Cr = (T)c.re, Ci = (T)c.im;
+        stage_h(0);
+        deriv(P, Q, Cr, Ci);
+        T ap = kp, ar = kr, ai = ki;
+        stage_h(1);
+        deriv(fmaT(half, kp, P), fmaT(-half, kp, Q), fmaT(half, kr, Cr), fmaT(half, ki, Ci));
+        ap = fmaT((T)2, kp, ap); ar = fmaT((T)2, kr, ar); ai = fmaT((T)2, ki, ai);
+        deriv(fmaT(half, kp, P), fmaT(-half, kp, Q), fmaT(half, kr, Cr), fmaT(half, ki, Ci));
+        ap = fmaT((T)2, kp, ap); ar = fmaT((T)2, kr, ar); ai = fmaT((T)2, ki, ai);
+        stage_h(2);
+        deriv(fmaT(full, kp, P), fmaT(-full, kp, Q), fmaT(full, kr, Cr), fmaT(full, ki, Ci));
+        ap += kp; ar += kr; ai += ki;
+        p = fmaT(pref, (double)ap, p);
+        q = fmaT(-pref, (double)ap, q);
+        c.re = fmaT(pref, (double)ar, c.re);
+        c.im = fmaT(pref, (double)ai, c.im);
+        double jc[C];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {  // inactive lanes hold rho = 0 and contribute 0
+            jc[2 * v] = fmaT(mr[v][3], c.im, fmaT(mr[v][2], c.re, fmaT(mr[v][1], q, mr[v][0] * p)));
+            jc[2 * v + 1] = fmaT(mi[v][3], c.im, fmaT(mi[v][2], c.re, fmaT(mi[v][1], q, mi[v][0] * p)));
+        }
+        JS::put(jl, lane, ti % TCH, jc);
+        if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
     }
     if (active) {
         double* o = rho_out + 2 * (size_t)k;
@@ -675,6 +827,15 @@ __global__ __launch_bounds__(64) void k_pair2_batch(const DevInst* __restrict__ 
     const int2 t = tiles[blockIdx.x];
     pair2_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
 }
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_pair2s(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
+    pair2s_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
+}
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_pair2s_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    pair2s_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
+}
 template <int EPL, typename T>
 __global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const InstPtrs ip) {
     elem_body<EPL, T>(d, ip, blockIdx.x);
@@ -686,12 +847,13 @@ __global__ __launch_bounds__(256) void k_elem_per_lane_batch(const DevInst* __re
     elem_body<EPL, T>(insts[t.x].d, insts[t.x].p, t.y);
 }
 
-// J[ti][c] = (sum over waves, in wave order per lane stripe) / N. The
-// partials carry C components per wave: C = 6 (re/im of x, y, z) or C = 3
-// (real parts of the current along the axes ax0, ax1, ax2 -- the Hermitian
-// path, whose current is real; unused slots are zero).
-__global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, int C, int ax0,
-                                                 int ax1, int ax2, double N, double* __restrict__ J) {
+// J[ti][omap[c]] = (sum over waves of component c, in a fixed order) / N;
+// output slots are re/im of x, y, z (0..5), unused slots are zero.
+struct OMap {
+    int o[6];
+};
+__global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, int C, OMap om,
+                                                 double N, double* __restrict__ J) {
     const int ti = blockIdx.x;
     const int lane = threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
@@ -702,18 +864,12 @@ __global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpar
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
     if (lane == 0) {
-        double* out = J + (size_t)ti * 6;
-        if (C == 6) {
+        double out[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = acc[c] / N;
-        } else {
+        for (int c = 0; c < 6; ++c)
+            if (c < C && om.o[c] >= 0) out[om.o[c]] = acc[c] / N;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = 0.0;
-            const int ax[3] = {ax0, ax1, ax2};
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (ax[c] >= 0) out[2 * ax[c]] = acc[c] / N;
-        }
+        for (int c = 0; c < 6; ++c) J[(size_t)ti * 6 + c] = out[c];
     }
 }
 
@@ -730,18 +886,13 @@ __global__ __launch_bounds__(64) void k_reduce_j_batch(const DevInst* __restrict
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] = wave_sum(acc[c]);
     if (lane == 0) {
-        double* out = I.J + (size_t)t.y * 6;
         const double N = (double)I.d.N;
-        if (C == 6) {
+        double out[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = acc[c] / N;
-        } else {
+        for (int c = 0; c < 6; ++c)
+            if (c < C && I.omap[c] >= 0) out[I.omap[c]] = acc[c] / N;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                if (I.axis[c] >= 0) out[2 * I.axis[c]] = acc[c] / N;
-        }
+        for (int c = 0; c < 6; ++c) I.J[(size_t)t.y * 6 + c] = out[c];
     }
 }
 
@@ -752,7 +903,7 @@ struct DeviceSolve {
     double *blk = nullptr, *steptab = nullptr, *tw1 = nullptr, *tw2 = nullptr;
     double *rho = nullptr, *rho_tmp = nullptr, *jpart = nullptr, *J = nullptr;
     PairDesc pd{};
-    double *pcoef = nullptr, *psteps = nullptr;
+    double *pcoef = nullptr, *scoef = nullptr, *psteps = nullptr;
     double* hJ = nullptr;  // pinned host copy of J for asynchronous completion
     int last_C = 6;
     size_t jpart_bytes = 0;
@@ -795,7 +946,8 @@ DeviceSolve* dev_create(const HostSolve& hs) {
         ds->rho = upload(hs.rho0, ds->stream);
         ds->pd = hs.pd;
         if (hs.pd.nrep > 0) {
-            ds->pcoef = upload(hs.pcoef, ds->stream);
+            if (!hs.pcoef.empty()) ds->pcoef = upload(hs.pcoef, ds->stream);
+            ds->scoef = upload(hs.scoef, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
         }
         ds->rho_tmp = dalloc(sizeof(double) * hs.rho0.size(), ds->stream);
@@ -812,7 +964,7 @@ DeviceSolve* dev_create(const HostSolve& hs) {
 void dev_destroy(DeviceSolve* ds) {
     if (!ds) return;
     for (double* p : {ds->blk, ds->steptab, ds->tw1, ds->tw2, ds->rho, ds->rho_tmp, ds->jpart, ds->J, ds->pcoef,
-                      ds->psteps})
+                      ds->scoef, ds->psteps})
         if (p && ds->stream) (void)hipFreeAsync(p, ds->stream);
     if (ds->stream) (void)hipStreamSynchronize(ds->stream);
     if (ds->hJ) (void)hipHostFree(ds->hJ);
@@ -831,7 +983,8 @@ size_t dev_bytes(const HostSolve& hs) {
     while (G < E && G < 64) G <<= 1;
     const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
     return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * hs.rho0.size() +
-                             hs.pcoef.size() + hs.psteps.size() + 6 * (size_t)d.nt + 6 * (size_t)d.nt * waves);
+                             hs.pcoef.size() + hs.scoef.size() + hs.psteps.size() + 6 * (size_t)d.nt +
+                             6 * (size_t)d.nt * waves);
 }
 
 static int pow2ceil(int n) {
@@ -841,7 +994,7 @@ static int pow2ceil(int n) {
 }
 
 // Which kernel and template instance a solve runs with.
-enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3 };
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4 };
 struct LaunchKey {
     int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
     bool operator<(const LaunchKey& o) const {
@@ -854,6 +1007,14 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
     const PairDesc& pd = ds->pd;
     LaunchKey k;
     k.fp32 = precision != 0;
+    if (pd.nrep > 0 && ds->use_herm && !(pd.vpaired && pd.nrep <= 4)) {
+        k.kind = KIND_PAIRS;
+        k.a = pd.nrep <= 12 ? 12 : 32;
+        if (pd.nx == 0 && pd.nv <= 2) { k.b = 0; k.c = 2; }
+        else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
+        else { k.b = 3; k.c = 3; }
+        return k;
+    }
     if (pd.nrep > 0 && pd.nrep <= 4 && ds->use_herm) {
         k.kind = KIND_PAIR;
         k.a = pd.nrep <= 3 ? 3 : 4;
@@ -874,7 +1035,21 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
     return k;
 }
 
-static int comps_of(const LaunchKey& k) { return k.kind == KIND_PAIR ? 3 : 6; }
+static int comps_of(const LaunchKey& k) {
+    if (k.kind == KIND_PAIR) return 3;
+    if (k.kind == KIND_PAIRS) return 2 * k.c;
+    return 6;
+}
+
+static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
+    OMap m;
+    for (int c = 0; c < 6; ++c) m.o[c] = (k.kind == KIND_PAIR || k.kind == KIND_PAIRS) ? -1 : c;
+    if (k.kind == KIND_PAIR)
+        for (int v = 0; v < ds->pd.nv; ++v) m.o[v] = 2 * ds->pd.vaxis[v];
+    if (k.kind == KIND_PAIRS)
+        for (int v = 0; v < ds->pd.nv; ++v) { m.o[2 * v] = 2 * ds->pd.vaxis[v]; m.o[2 * v + 1] = 2 * ds->pd.vaxis[v] + 1; }
+    return m;
+}
 
 // Tiles (workgroups) of one instance and waves per tile.
 static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block) {
@@ -915,6 +1090,14 @@ static void dispatch(const LaunchKey& k, F&& f) {
             PAIR(3, 2, 2) PAIR(4, 2, 2) PAIR(3, 3, 3) PAIR(4, 3, 3)
 #undef PAIR
             break;
+        case KIND_PAIRS:
+#define PAIRS(R, X, V)                                                                                   \
+    if (k.a == R && k.b == X && k.c == V)                                                                \
+        return go(k_pair2s<R, X, V, double>, k_pair2s_batch<R, X, V, double>, k_pair2s<R, X, V, float>,  \
+                  k_pair2s_batch<R, X, V, float>);
+            PAIRS(12, 0, 2) PAIRS(12, 2, 2) PAIRS(12, 3, 3) PAIRS(32, 0, 2) PAIRS(32, 2, 2) PAIRS(32, 3, 3)
+#undef PAIRS
+            break;
         case KIND_THREAD:
 #define THR(NB, U)                                                                                           \
     if (k.a == NB && k.b == U)                                                                               \
@@ -948,7 +1131,7 @@ static void ensure_jpart(DeviceSolve* ds, int nwaves, int C) {
 static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves) {
     InstPtrs p;
     p.blk = ds->blk; p.steptab = ds->steptab; p.tw1 = ds->tw1; p.tw2 = ds->tw2;
-    p.pcoef = ds->pcoef; p.psteps = ds->psteps; p.rho_in = ds->rho;
+    p.pcoef = ds->pcoef; p.scoef = ds->scoef; p.psteps = ds->psteps; p.rho_in = ds->rho;
     p.rho_out = ds->rho_tmp; p.jpart = ds->jpart; p.nwaves = nwaves; p.dt = ds->dt;
     return p;
 }
@@ -986,10 +1169,8 @@ void dev_launch(DeviceSolve* ds, int precision) {
     HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
     const int C = launch_single(ds, precision, nwaves);
     HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
-    int ax[3] = {-1, -1, -1};
-    for (int v = 0; v < ds->pd.nv && C == 3; ++v) ax[v] = ds->pd.vaxis[v];
-    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, C, ax[0], ax[1], ax[2],
-                       (double)d.N, ds->J);
+    const OMap om = omap_of(ds, classify(ds, precision));
+    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ds->jpart, nwaves, C, om, (double)d.N, ds->J);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(ds->hJ, ds->J, sizeof(double) * 6 * (size_t)d.nt, hipMemcpyDeviceToHost, ds->stream));
     HIP_CHECK(hipEventRecord(ds->ev1, ds->stream));
@@ -1110,7 +1291,8 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
             I.p = ptrs_of(ds, nwaves);
             I.J = ds->J;
             I.C = comps_of(key);
-            for (int c = 0; c < 3; ++c) I.axis[c] = (I.C == 3 && c < ds->pd.nv) ? ds->pd.vaxis[c] : -1;
+            const OMap om = omap_of(ds, key);
+            for (int c = 0; c < 6; ++c) I.omap[c] = om.o[c];
             lds = std::max(lds, lds_of(ds->d, key));
             for (int t = 0; t < ds->d.nt; ++t) rt.push_back(make_int2((int)j, t));
         }
@@ -1173,10 +1355,11 @@ int dev_kernel_variant(DeviceSolve* ds) { return classify(ds, 0).kind; }
 // current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8).
 double dev_flops_per_kstep(DeviceSolve* ds) {
     const SolveDesc& d = ds->d;
-    if (dev_kernel_variant(ds) == 2) {
+    const int kind = dev_kernel_variant(ds);
+    if (kind == KIND_PAIR || kind == KIND_PAIRS) {
         const PairDesc& pd = ds->pd;
-        return 6.0 * pd.nrep + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 +
-               pd.nv * (16.0 * pd.nrep + 7);
+        const double jcost = kind == KIND_PAIR ? pd.nv * (16.0 * pd.nrep + 7) : pd.nv * (32.0 * pd.nrep + 14);
+        return 6.0 * pd.nrep + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 + jcost;
     }
     int nv = 0;
     for (int a = 0; a < 3; ++a) nv += (d.vmask >> a) & 1;

@@ -301,27 +301,32 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
                 if (q.om & (1u << (OP_X + ax))) xm |= 1 << ax;
         d.xmask &= xm;
     }
-    // Hermitian pairing: B(-n) = B(n)^dagger for every op (to 1e-13 of the
-    // block scale, which covers the rounding of the convolved velocity) and
-    // every partner inside the supercell window the op lives on.
-    d.herm = 1;
-    for (size_t i = 0; i < sup.size() && d.herm; ++i) {
-        const Sup& a = sup[i];
-        const Sup* b = nullptr;
-        for (auto& q : sup) if (q.n1 == -a.n1 && q.n2 == -a.n2) b = &q;
-        for (int o = 0; o < NOPS && d.herm; ++o) {
-            const cplx* A = &a.b[(size_t)o * nb2];
-            double scale = 0.0;
-            for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(A[e]));
-            if (scale == 0.0 && !(b && (b->om & (1u << o)))) continue;
-            if (!b) { d.herm = 0; break; }
-            const cplx* B = &b->b[(size_t)o * nb2];
-            for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(B[e]));
-            for (int i1 = 0; i1 < nb; ++i1)
-                for (int j1 = 0; j1 < nb; ++j1)
-                    if (std::abs(B[j1 * nb + i1] - std::conj(A[i1 * nb + j1])) > 1e-13 * scale) d.herm = 0;
+    // Hermitian pairing: B(-n) = B(n)^dagger (to 1e-13 of the block scale,
+    // which covers the rounding of the convolved velocity), checked for the
+    // H and rbc blocks and, separately, for the v blocks of the window.
+    auto pair_ok = [&](int o_lo, int o_hi) {
+        for (size_t i = 0; i < sup.size(); ++i) {
+            const Sup& a = sup[i];
+            const Sup* b = nullptr;
+            for (auto& q : sup) if (q.n1 == -a.n1 && q.n2 == -a.n2) b = &q;
+            for (int o = o_lo; o < o_hi; ++o) {
+                const cplx* A = &a.b[(size_t)o * nb2];
+                double scale = 0.0;
+                for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(A[e]));
+                if (scale == 0.0 && !(b && (b->om & (1u << o)))) continue;
+                if (!b) return false;
+                const cplx* B = &b->b[(size_t)o * nb2];
+                for (int e = 0; e < nb2; ++e) scale = std::max(scale, std::abs(B[e]));
+                for (int i1 = 0; i1 < nb; ++i1)
+                    for (int j1 = 0; j1 < nb; ++j1)
+                        if (std::abs(B[j1 * nb + i1] - std::conj(A[i1 * nb + j1])) > 1e-13 * scale) return false;
+            }
         }
-    }
+        return true;
+    };
+    const bool herm_hx = pair_ok(OP_H, OP_V);
+    const bool vpaired = pair_ok(OP_V, NOPS);
+    d.herm = herm_hx && vpaired;
     const int ns = (int)sup.size();
     d.ns = ns;
     d.stride = STEP_HEAD + 2 * ns;
@@ -358,60 +363,127 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
         }
     }
 
-    // Pair pack for the Hermitian nb = 2 kernel.
+    // Pair packs of the Hermitian nb = 2 kernels. Pairs are represented by
+    // their canonical point (n1 > 0, or n1 == 0 and n2 > 0).
     hs.pd = PairDesc{};
-    if (nb == 2 && d.herm) {
+    if (nb == 2 && herm_hx) {
         PairDesc& pd = hs.pd;
         for (int ax = 0; ax < 3; ++ax) {
             if (d.xmask & (1 << ax)) pd.xaxis[pd.nx++] = ax;
             if (d.vmask & (1 << ax)) pd.vaxis[pd.nv++] = ax;
         }
-        std::vector<int> reps;
-        for (int q = 0; q < ns; ++q)
-            if (d.role[q] == ROLE_REP) reps.push_back(q);
-        if ((int)reps.size() <= PAIR_MAX) {
+        std::vector<std::pair<int, int>> reps;
+        for (auto& q : sup) {
+            if (q.n1 == 0 && q.n2 == 0) continue;
+            const bool canon = q.n1 > 0 || (q.n1 == 0 && q.n2 > 0);
+            const std::pair<int, int> c = canon ? std::make_pair(q.n1, q.n2) : std::make_pair(-q.n1, -q.n2);
+            if (std::find(reps.begin(), reps.end(), c) == reps.end()) reps.push_back(c);
+        }
+        if ((int)reps.size() <= PAIR_MAX && pd.nv > 0) {
             pd.nrep = (int)reps.size();
-            hs.pcoef.assign(PAIR_HEAD + PAIR_MAX * PAIR_REP, 0.0);
-            auto blk_of = [&](int q, int o) { return &sup[q].b[(size_t)o * 4]; };
-            auto origin = [&](int o, double* out, double scale01) {
-                for (int q = 0; q < ns; ++q)
-                    if (d.role[q] == ROLE_ORIGIN) {
-                        const cplx* B = blk_of(q, o);
-                        out[0] = B[0].real(); out[1] = B[3].real();
-                        out[2] = scale01 * B[1].real(); out[3] = scale01 * B[1].imag();
-                    }
+            pd.vpaired = vpaired ? 1 : 0;
+            auto at = [&](int n1, int n2) -> const Sup* {
+                for (auto& q : sup) if (q.n1 == n1 && q.n2 == n2) return &q;
+                return nullptr;
             };
-            auto pair = [&](int q, int o, double* out, double scale01) {
-                const cplx* B = blk_of(q, o);
+            static const cplx zero4[4] = {0.0, 0.0, 0.0, 0.0};
+            auto blk_at = [&](const Sup* q, int o) -> const cplx* { return q ? &q->b[(size_t)o * 4] : zero4; };
+            // H / rbc pair coefficients: 4 real outputs (h00, h11, Re h01, Im h01)
+            auto pair_hx = [&](const cplx* B, double* out) {
                 const cplx b00 = B[0], b01 = B[1], b10 = B[2], b11 = B[3];
                 out[0] = 2 * b00.real(); out[1] = -2 * b00.imag();
                 out[2] = 2 * b11.real(); out[3] = -2 * b11.imag();
-                out[4] = scale01 * (b01.real() + b10.real()); out[5] = scale01 * (-b01.imag() - b10.imag());
-                out[6] = scale01 * (b01.imag() - b10.imag()); out[7] = scale01 * (b01.real() - b10.real());
+                out[4] = b01.real() + b10.real(); out[5] = -b01.imag() - b10.imag();
+                out[6] = b01.imag() - b10.imag(); out[7] = b01.real() - b10.real();
             };
-            double* c = hs.pcoef.data();
-            origin(OP_H, c, 1.0);
-            for (int x = 0; x < pd.nx; ++x) origin(OP_X + pd.xaxis[x], c + 4 + 4 * x, 1.0);
-            for (int v = 0; v < pd.nv; ++v) origin(OP_V + pd.vaxis[v], c + 16 + 4 * v, 2.0);
-            for (int r = 0; r < pd.nrep; ++r) {
-                double* cr = c + PAIR_HEAD + PAIR_REP * r;
-                pd.n1[r] = sup[reps[r]].n1; pd.n2[r] = sup[reps[r]].n2;
-                pair(reps[r], OP_H, cr, 1.0);
-                for (int x = 0; x < pd.nx; ++x) pair(reps[r], OP_X + pd.xaxis[x], cr + 8 + 8 * x, 1.0);
-                for (int v = 0; v < pd.nv; ++v) pair(reps[r], OP_V + pd.vaxis[v], cr + 32 + 8 * v, 2.0);
+            auto origin_hx = [&](const cplx* B, double* out) {
+                out[0] = B[0].real(); out[1] = B[3].real(); out[2] = B[1].real(); out[3] = B[1].imag();
+            };
+            // current: u(rho) = L . (p, q, Re c, Im c), L = (V00, V11, V01 + V10, i (V10 - V01))
+            auto Lvec = [](const cplx* V, cplx* L) {
+                L[0] = V[0]; L[1] = V[3]; L[2] = V[1] + V[2]; L[3] = cplx(0.0, 1.0) * (V[2] - V[1]);
+            };
+            const Sup* org = at(0, 0);
+            // streamed pack
+            hs.scoef.assign(PS_HEAD + (size_t)PAIR_MAX * PS_REP, 0.0);
+            double* sc = hs.scoef.data();
+            origin_hx(blk_at(org, OP_H), sc);
+            for (int x = 0; x < pd.nx; ++x) origin_hx(blk_at(org, OP_X + pd.xaxis[x]), sc + 4 + 4 * x);
+            for (int v = 0; v < pd.nv; ++v) {
+                cplx L[4];
+                Lvec(blk_at(org, OP_V + pd.vaxis[v]), L);
+                for (int j = 0; j < 4; ++j) { sc[16 + 8 * v + j] = L[j].real(); sc[16 + 8 * v + 4 + j] = L[j].imag(); }
             }
-            pd.stride = PAIR_STEP_HEAD + 2 * PAIR_MAX;  // rows padded: unused pairs read phase 0
+            for (int r = 0; r < pd.nrep; ++r) {
+                const int n1 = reps[r].first, n2 = reps[r].second;
+                pd.n1[r] = n1; pd.n2[r] = n2;
+                const Sup* P = at(n1, n2);
+                const Sup* M = at(-n1, -n2);
+                double* cr = sc + PS_HEAD + PS_REP * r;
+                // H and rbc from the representative (the partner holds its dagger)
+                const Sup* src = P ? P : M;
+                const bool from_partner = (P == nullptr);
+                auto hx_block = [&](int o, cplx* B) {
+                    const cplx* b = blk_at(src, o);
+                    if (!from_partner) { for (int e = 0; e < 4; ++e) B[e] = b[e]; return; }
+                    B[0] = std::conj(b[0]); B[1] = std::conj(b[2]); B[2] = std::conj(b[1]); B[3] = std::conj(b[3]);
+                };
+                cplx B[4];
+                hx_block(OP_H, B);
+                pair_hx(B, cr);
+                for (int x = 0; x < pd.nx; ++x) { hx_block(OP_X + pd.xaxis[x], B); pair_hx(B, cr + 8 + 8 * x); }
+                for (int v = 0; v < pd.nv; ++v) {
+                    cplx L[4], Lp[4];
+                    Lvec(blk_at(P, OP_V + pd.vaxis[v]), L);    // point +n: weight w
+                    Lvec(blk_at(M, OP_V + pd.vaxis[v]), Lp);   // point -n: weight conj(w)
+                    double* cv = cr + 32 + 16 * v;
+                    for (int j = 0; j < 4; ++j) {
+                        cv[2 * j] = L[j].real() + Lp[j].real();       // Re J, * Re w
+                        cv[2 * j + 1] = -L[j].imag() + Lp[j].imag();  // Re J, * Im w
+                        cv[8 + 2 * j] = L[j].imag() + Lp[j].imag();   // Im J, * Re w
+                        cv[8 + 2 * j + 1] = L[j].real() - Lp[j].real();  // Im J, * Im w
+                    }
+                }
+            }
+            // register pack (real current) when the v blocks pair as well
+            if (vpaired) {
+                hs.pcoef.assign(PAIR_HEAD + (size_t)PAIR_MAX * PAIR_REP, 0.0);
+                double* c = hs.pcoef.data();
+                for (int j = 0; j < 16; ++j) c[j] = sc[j];
+                for (int v = 0; v < pd.nv; ++v)  // v origin: (V00, V11, 2 Re V01, 2 Im V01) = Re L for Hermitian V(0)
+                    for (int j = 0; j < 4; ++j) c[16 + 4 * v + j] = sc[16 + 8 * v + j];
+                for (int r = 0; r < pd.nrep; ++r) {
+                    const double* cr = sc + PS_HEAD + PS_REP * r;
+                    double* o = c + PAIR_HEAD + PAIR_REP * r;
+                    for (int j = 0; j < 32; ++j) o[j] = cr[j];
+                    for (int v = 0; v < pd.nv; ++v)
+                        for (int j = 0; j < 8; ++j) o[32 + 8 * v + j] = cr[32 + 16 * v + j];
+                }
+            }
+            // per-step rows: stage fields of the rbc slots and e^{i A.R} of the pair points
+            pd.stride = PAIR_STEP_HEAD + 2 * PAIR_MAX;
             hs.psteps.assign((size_t)nt * pd.stride, 0.0);
+            std::vector<double> Rr(3 * (size_t)pd.nrep);
+            for (int r = 0; r < pd.nrep; ++r)
+                for (int ax = 0; ax < 3; ++ax) {
+                    double v = RA(pd.n1[r], pd.n2[r], ax);
+                    v *= 1.0 / 0.5291772;
+                    Rr[3 * r + ax] = v;
+                }
             for (int ti = 0; ti < nt; ++ti) {
                 const double* row = &hs.steptab[(size_t)ti * d.stride];
                 double* out = &hs.psteps[(size_t)ti * pd.stride];
                 for (int j = 0; j < 3; ++j)
                     for (int x = 0; x < pd.nx; ++x) out[3 * j + x] = row[3 * j + pd.xaxis[x]];
+                const double ax_ = hs.A[0][ti], ay_ = hs.A[1][ti], az_ = hs.A[2][ti];
                 for (int r = 0; r < pd.nrep; ++r) {
-                    out[PAIR_STEP_HEAD + 2 * r] = row[STEP_HEAD + 2 * reps[r]];
-                    out[PAIR_STEP_HEAD + 2 * r + 1] = row[STEP_HEAD + 2 * reps[r] + 1];
+                    const cplx ph = std::exp(cplx(0.0, +ax_ * Rr[3 * r] + ay_ * Rr[3 * r + 1] + az_ * Rr[3 * r + 2]));
+                    out[PAIR_STEP_HEAD + 2 * r] = ph.real();
+                    out[PAIR_STEP_HEAD + 2 * r + 1] = ph.imag();
                 }
             }
+        } else {
+            pd = PairDesc{};
         }
     }
 

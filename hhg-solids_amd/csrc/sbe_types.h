@@ -50,20 +50,30 @@ struct SolveDesc {
 // pre-scaled so that J = m0 p + m1 q + m2 Re(c) + m3 Im(c) with rho =
 // [p c; c* q]. steps rows: 9 stage fields (3 stages x 3 rbc slots), 1 pad,
 // then e^{i A(t_i).R_r} per pair r (complex).
-constexpr int PAIR_MAX = 16;
+constexpr int PAIR_MAX = 32;
 constexpr int PAIR_HEAD = 28;
 constexpr int PAIR_REP = 56;
 constexpr int PAIR_STEP_HEAD = 10;
 
+// Streamed pack (k_pair2s): H and rbc pairs as above; the current may be
+// complex (velocity blocks whose -R partner lies outside the 4x4 window).
+// With rho_vec = (p, q, Re c, Im c), J = sum_j rho_vec_j (m_j + i m'_j) and
+// m_j = o_j + sum_r (a_rj Re w + b_rj Im w), m'_j likewise. Layout (doubles):
+//   [0, 4) H0, [4, 16) rbc0 slots, [16, 40) v0 slots: (o_0..3, o'_0..3)
+//   PS_HEAD + PS_REP r: H (8), rbc slots (3 x 8), v slots (3 x 16: a0 b0 .. a3 b3, a'0 b'0 .. a'3 b'3)
+constexpr int PS_HEAD = 40;
+constexpr int PS_REP = 80;
+
 struct PairDesc {
     int nrep, nx, nv, stride;
+    int vpaired;          // 1: every v block has its -R partner, the current is real
     int xaxis[3], vaxis[3];
     int n1[PAIR_MAX], n2[PAIR_MAX];
 };
 
 // Device pointers and scalars of one instance (solve) on the GPU.
 struct InstPtrs {
-    const double *blk, *steptab, *tw1, *tw2, *pcoef, *psteps, *rho_in;
+    const double *blk, *steptab, *tw1, *tw2, *pcoef, *scoef, *psteps, *rho_in;
     double *rho_out, *jpart;
     int nwaves;          // waves (current partials per step) of this instance's launch
     double dt;
@@ -75,8 +85,8 @@ struct DevInst {
     PairDesc pd;
     InstPtrs p;
     double* J;           // [nt][6] reduced current
-    int C;               // current components per wave partial (6, or 3 on the Hermitian path)
-    int axis[3];         // axes of the 3 components when C == 3 (-1 = unused)
+    int C;               // current components per wave partial
+    int omap[6];         // output slot (re/im of x, y, z = 0..5) of each component, -1 = unused
 };
 
 }  // namespace hhg
