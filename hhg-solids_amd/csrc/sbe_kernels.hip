@@ -15,13 +15,20 @@
 //   k_pair2<NREP,NX,NV,T>      nb = 2 with Hermitian blocks: one lane per k,
 //                              the +-R pairs folded into real coefficient
 //                              matrices, branch-free time loop.
+//   k_pair2s<NREP,NX,NV,T>     nb = 2 with Hermitian H and rbc, pair
+//                              coefficients streamed through scalar loads;
+//                              complex (window) current.
+//   k_sep<NBP,MAXD,T>          Hermitian 3 <= nb <= 8: row-separable Fourier
+//                              sums built once per k-row and step in LDS,
+//                              2x2 element blocks per lane, M - M^dagger
+//                              commutator.
 //   k_thread_per_k<NB,MAXU,T>  one lane per k, everything in registers; the
 //                              blocks and the per-step table are uniform and
 //                              arrive through scalar loads. For nb <= 2.
 //   k_elem_per_lane<EPL,T>     G = pow2(nb*nb) <= 64 lanes per k, each lane
 //                              owns EPL matrix elements; the stage
 //                              Hamiltonian and stage rho are staged through
-//                              LDS for the commutator. Any nb <= 16.
+//                              LDS for the commutator. Any model, nb <= 16.
 //   k_reduce_j                 deterministic sum of the per-wave partials.
 // T is the stage arithmetic (double for the plain path, float for the mixed
 // path); the state rho and the current are always FP64.
@@ -804,6 +811,312 @@ __device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip
         }
 }
 
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// --------------------------------------------------------------------------
+// Row-separable element kernel: Hermitian H and rbc, 3 <= nb <= NBP (4 or 8,
+// smaller nb zero-padded). A workgroup of W waves owns a chunk of one k-row
+// m1. Per step it builds, cooperatively and double-buffered in LDS,
+//   G[j][d]  = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} (H_s + E_j . rbc_s)   (stage fields j = 0..2)
+//   GV[w][v] = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} v_s                  (values d with a v block)
+// and every k-point forms its stage Hamiltonian sum_d tw2[m2 n2_d] G[j][d]
+// with nd complex MACs per element instead of ns. For Hermitian H and X the
+// commutator is -i(M - M^dagger) with M = H X: one complex product per stage.
+// Each lane owns a 2x2 element block, so every LDS read of the product feeds
+// two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and exchange H,
+// X and M through LDS with wave-level ordering only. One workgroup barrier per
+// step (the G tables of step t+1 are built while step t runs).
+struct SepLayout {
+    size_t off_g, gbytes, off_gv, gvbytes, off_k, kbytes, off_jw, bytes;
+    int nv;
+};
+__host__ __device__ inline SepLayout sep_layout(const SolveDesc& d, int NBP, int tsize, int W) {
+    SepLayout L;
+    L.nv = 0;
+    for (int a = 0; a < 3; ++a) L.nv += (d.vmask >> a) & 1;
+    const size_t E = (size_t)NBP * NBP;
+    const int GPW = 64 / ((NBP / 2) * (NBP / 2));
+    size_t o = 0;
+    L.gbytes = 3 * (size_t)d.nd * E * 2 * tsize;
+    L.off_g = o; o = align16(o + 2 * L.gbytes);
+    L.gvbytes = (size_t)d.nwd * L.nv * E * 16;
+    L.off_gv = o; o = align16(o + 2 * L.gvbytes);
+    L.kbytes = 2 * E * 2 * tsize + 16;  // H/M and X of one k-point; the pad staggers banks
+    L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
+    L.off_jw = o; o = align16(o + sizeof(double) * 2 * W * 6);
+    L.bytes = o;
+    return L;
+}
+
+// Orders the LDS accesses of one wave: writes before the barrier are seen by
+// every lane's reads after it (LDS executes a wave's instructions in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Two consecutive complex values (16-byte aligned pairs).
+__device__ __forceinline__ void ld2(const cx<float>* p, cx<float>& a, cx<float>& b) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    a = {v.x, v.y};
+    b = {v.z, v.w};
+}
+__device__ __forceinline__ void ld2(const cx<double>* p, cx<double>& a, cx<double>& b) {
+    const double2 v0 = reinterpret_cast<const double2*>(p)[0];
+    const double2 v1 = reinterpret_cast<const double2*>(p)[1];
+    a = {v0.x, v0.y};
+    b = {v1.x, v1.y};
+}
+__device__ __forceinline__ void st2(cx<float>* p, cx<float> a, cx<float> b) {
+    *reinterpret_cast<float4*>(p) = make_float4(a.re, a.im, b.re, b.im);
+}
+__device__ __forceinline__ void st2(cx<double>* p, cx<double> a, cx<double> b) {
+    reinterpret_cast<double2*>(p)[0] = make_double2(a.re, a.im);
+    reinterpret_cast<double2*>(p)[1] = make_double2(b.re, b.im);
+}
+template <typename T>
+__device__ __forceinline__ void cfma(cx<T>& acc, cx<T> a, cx<T> b) {  // acc += a*b
+    acc.re = fmaT(a.re, b.re, fmaT(-a.im, b.im, acc.re));
+    acc.im = fmaT(a.re, b.im, fmaT(a.im, b.re, acc.im));
+}
+template <typename T>
+__device__ __forceinline__ void cfmac(cx<T>& acc, cx<T> a, cx<T> b) {  // acc += conj(a)*b
+    acc.re = fmaT(a.re, b.re, fmaT(a.im, b.im, acc.re));
+    acc.im = fmaT(a.re, b.im, fmaT(-a.im, b.re, acc.im));
+}
+
+template <int NBP, int MAXD, typename T>
+__device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip, int tile) {
+    constexpr int HB = NBP / 2, LPK = HB * HB, GPW = 64 / LPK, E = NBP * NBP;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int W = blockDim.x >> 6;
+    const SepLayout L = sep_layout(d, NBP, (int)sizeof(T), W);
+    const int nb = d.nb, nbe = nb * nb, nd = d.nd, nv = L.nv, N = d.N, N1 = d.N1, N2 = d.N2;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int g = lane / LPK, bl = lane - g * LPK;
+    const int i0 = 2 * (bl / HB), j0 = 2 * (bl - (bl / HB) * HB);
+    const int kpt = W * GPW;
+    const int tpr = (N2 + kpt - 1) / kpt;
+    const int m1 = tile / tpr;
+    const int m2 = (tile - m1 * tpr) * kpt + wave * GPW + g;
+    const bool kval = m2 < N2;
+    const int k = m1 * N2 + (kval ? m2 : 0);
+    int xax[3] = {0, 0, 0}, vax[3] = {0, 0, 0}, nx = 0;
+    {
+        int c = 0;
+        for (int a = 0; a < 3; ++a) {
+            if ((d.xmask >> a) & 1) xax[nx++] = a;
+            if ((d.vmask >> a) & 1) vax[c++] = a;
+        }
+    }
+    cx<T>* bufH = reinterpret_cast<cx<T>*>(smem + L.off_k + (size_t)(wave * GPW + g) * L.kbytes);  // H, then M
+    cx<T>* bufX = bufH + E;
+    double* jw = reinterpret_cast<double*>(smem + L.off_jw);
+
+    // zero the tables once: padded elements are never written
+    for (size_t i = tid; i < L.off_k / 16; i += bd) reinterpret_cast<double2*>(smem)[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+
+    auto build = [&](int ti, int buf) {
+        const double* st = ip.steptab + (size_t)ti * d.stride;
+        cx<T>* G = reinterpret_cast<cx<T>*>(smem + L.off_g + buf * L.gbytes);
+        cx<double>* GV = reinterpret_cast<cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
+        for (int q = tid; q < nd * nbe; q += bd) {
+            const int dd = q / nbe, el = q - dd * nbe;
+            const int ep = (el / nb) * NBP + (el - (el / nb) * nb);
+            cx<double> a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
+            cx<double> av[3] = {{0, 0}, {0, 0}, {0, 0}};
+            for (int qq = d.dseg[dd]; qq < d.dseg[dd + 1]; ++qq) {
+                const int s = d.sorder[qq];
+                const uint32_t om = d.opmask[s];
+                const cx<double> w = cmul(ld<double>(ip.tw1 + 2 * posmod((long)m1 * d.n1[s], N1)),
+                                          ld<double>(st + STEP_HEAD + 2 * s));
+                const double* bs = ip.blk + 2 * ((size_t)s * NOPS * nbe + el);
+                const cx<double> h = (om & (1u << OP_H)) ? ld<double>(bs) : cx<double>{0, 0};
+                cx<double> b0 = h, b1 = h, b2 = h;
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    if (x >= nx || !(om & (1u << (OP_X + xax[x])))) continue;
+                    const int ax = xax[x];
+                    const cx<double> X = ld<double>(bs + 2 * (size_t)(OP_X + ax) * nbe);
+                    b0.re = fmaT(st[ax], X.re, b0.re); b0.im = fmaT(st[ax], X.im, b0.im);
+                    b1.re = fmaT(st[3 + ax], X.re, b1.re); b1.im = fmaT(st[3 + ax], X.im, b1.im);
+                    b2.re = fmaT(st[6 + ax], X.re, b2.re); b2.im = fmaT(st[6 + ax], X.im, b2.im);
+                }
+                cfma(a0, w, b0);
+                cfma(a1, w, b1);
+                cfma(a2, w, b2);
+#pragma unroll
+                for (int v = 0; v < 3; ++v)
+                    if (v < nv && (om & (1u << (OP_V + vax[v]))))
+                        cfma(av[v], w, ld<double>(bs + 2 * (size_t)(OP_V + vax[v]) * nbe));
+            }
+            G[(size_t)(0 * nd + dd) * E + ep] = cast<T>(a0);
+            G[(size_t)(1 * nd + dd) * E + ep] = cast<T>(a1);
+            G[(size_t)(2 * nd + dd) * E + ep] = cast<T>(a2);
+            const int wv = d.wdi[dd];
+            if (wv >= 0) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v)
+                    if (v < nv) GV[((size_t)wv * nv + v) * E + ep] = av[v];
+            }
+        }
+    };
+
+    cx<double> rho[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+        rho[q] = (kval && ei < nb && ej < nb) ? ld<double>(ip.rho_in + 2 * ((size_t)(ei * nb + ej) * N + k))
+                                              : cx<double>{0.0, 0.0};
+    }
+    cx<double> z[MAXD];
+#pragma unroll
+    for (int dd = 0; dd < MAXD; ++dd)
+        z[dd] = (dd < nd && kval) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2)) : cx<double>{0.0, 0.0};
+
+    const T half = (T)(0.5 * ip.dt);
+    const T full = (T)ip.dt;
+    const double pref = 0.1666666666666 * ip.dt;
+    build(0, 0);
+    __syncthreads();
+    for (int ti = 0; ti < d.nt; ++ti) {
+        const int buf = ti & 1;
+        if (ti + 1 < d.nt) build(ti + 1, buf ^ 1);
+        const cx<T>* G = reinterpret_cast<const cx<T>*>(smem + L.off_g + buf * L.gbytes);
+        const cx<double>* GV = reinterpret_cast<const cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
+        cx<T> Hs[4], rs[4], K[4], acc[4];
+        auto hstage = [&](int j) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Hs[q] = {(T)0, (T)0};
+#pragma unroll
+            for (int dd = 0; dd < MAXD; ++dd) {
+                if (dd >= nd) continue;
+                const cx<T> zz = cast<T>(z[dd]);
+                const cx<T>* gp = G + (size_t)(j * nd + dd) * E + i0 * NBP + j0;
+                cx<T> g0, g1, g2, g3;
+                ld2(gp, g0, g1);
+                ld2(gp + NBP, g2, g3);
+                cfma(Hs[0], zz, g0);
+                cfma(Hs[1], zz, g1);
+                cfma(Hs[2], zz, g2);
+                cfma(Hs[3], zz, g3);
+            }
+        };
+        auto deriv = [&]() {
+            st2(bufH + i0 * NBP + j0, Hs[0], Hs[1]);
+            st2(bufH + (i0 + 1) * NBP + j0, Hs[2], Hs[3]);
+            st2(bufX + i0 * NBP + j0, rs[0], rs[1]);
+            st2(bufX + (i0 + 1) * NBP + j0, rs[2], rs[3]);
+            wave_lds_sync();
+            cx<T> M[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) M[q] = {(T)0, (T)0};
+#pragma unroll
+            for (int m = 0; m < NBP; ++m) {
+                cx<T> h0, h1, r0, r1;
+                ld2(bufH + m * NBP + i0, h0, h1);  // H[m][i0..i0+1] = conj(H[i0..i0+1][m])
+                ld2(bufX + m * NBP + j0, r0, r1);  // X[m][j0..j0+1]
+                cfmac(M[0], h0, r0);
+                cfmac(M[1], h0, r1);
+                cfmac(M[2], h1, r0);
+                cfmac(M[3], h1, r1);
+            }
+            wave_lds_sync();
+            st2(bufH + i0 * NBP + j0, M[0], M[1]);
+            st2(bufH + (i0 + 1) * NBP + j0, M[2], M[3]);
+            wave_lds_sync();
+            cx<T> t0, t1, u0, u1;
+            ld2(bufH + j0 * NBP + i0, t0, t1);        // M[j0][i0], M[j0][i0+1]
+            ld2(bufH + (j0 + 1) * NBP + i0, u0, u1);  // M[j0+1][i0], M[j0+1][i0+1]
+            // K_ij = -i (M_ij - conj(M_ji))
+            K[0] = {M[0].im + t0.im, t0.re - M[0].re};
+            K[1] = {M[1].im + u0.im, u0.re - M[1].re};
+            K[2] = {M[2].im + t1.im, t1.re - M[2].re};
+            K[3] = {M[3].im + u1.im, u1.re - M[3].re};
+            wave_lds_sync();
+        };
+        hstage(0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rs[q] = cast<T>(rho[q]);
+        deriv();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q] = K[q];
+            rs[q] = {fmaT(half, K[q].re, (T)rho[q].re), fmaT(half, K[q].im, (T)rho[q].im)};
+        }
+        hstage(1);
+        deriv();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q].re = fmaT((T)2, K[q].re, acc[q].re); acc[q].im = fmaT((T)2, K[q].im, acc[q].im);
+            rs[q] = {fmaT(half, K[q].re, (T)rho[q].re), fmaT(half, K[q].im, (T)rho[q].im)};
+        }
+        deriv();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q].re = fmaT((T)2, K[q].re, acc[q].re); acc[q].im = fmaT((T)2, K[q].im, acc[q].im);
+            rs[q] = {fmaT(full, K[q].re, (T)rho[q].re), fmaT(full, K[q].im, (T)rho[q].im)};
+        }
+        hstage(2);
+        deriv();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            acc[q].re += K[q].re; acc[q].im += K[q].im;
+            rho[q].re = fmaT(pref, (double)acc[q].re, rho[q].re);
+            rho[q].im = fmaT(pref, (double)acc[q].im, rho[q].im);
+        }
+        // current of this step: sum_d tw2[m2 n2_d] sum_ij GV[d][ij] conj(rho_ij)
+        cx<double> jv[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int dd = 0; dd < MAXD; ++dd) {
+            const int wv = dd < nd ? d.wdi[dd] : -1;
+            if (wv < 0) continue;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                if (v >= nv) continue;
+                const cx<double>* vp = GV + ((size_t)wv * nv + v) * E + i0 * NBP + j0;
+                cx<double> v0, v1, v2, v3;
+                ld2(vp, v0, v1);
+                ld2(vp + NBP, v2, v3);
+                cx<double> s = {0, 0};
+                cfmac(s, rho[0], v0);
+                cfmac(s, rho[1], v1);
+                cfmac(s, rho[2], v2);
+                cfmac(s, rho[3], v3);
+                cfma(jv[v], z[dd], s);
+            }
+        }
+        double jc[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int v = 0; v < 3; ++v)
+                if (v < nv && vax[v] == a) { jc[2 * a] = jv[v].re; jc[2 * a + 1] = jv[v].im; }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) jc[c] = wave_sum(jc[c]);
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) jw[(buf * W + wave) * 6 + c] = jc[c];
+        }
+        __syncthreads();
+        if (tid < 6) {
+            double sacc = 0.0;
+            for (int w = 0; w < W; ++w) sacc += jw[(buf * W + w) * 6 + tid];
+            ip.jpart[((size_t)ti * ip.nwaves + tile) * 6 + tid] = sacc;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+        if (kval && ei < nb && ej < nb) {
+            ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k)] = rho[q].re;
+            ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k) + 1] = rho[q].im;
+        }
+    }
+}
+
 // --------------------------------------------------------------------------
 // Launch wrappers. Single instance: descriptor in kernel arguments, block b
 // handles tile b. Batch: tiles[b] = (instance, tile of that instance), the
@@ -836,6 +1149,16 @@ __global__ __launch_bounds__(64) void k_pair2s_batch(const DevInst* __restrict__
     const int2 t = tiles[blockIdx.x];
     pair2s_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
 }
+template <int NBP, int MAXD, typename T>
+__global__ __launch_bounds__(512) void k_sep(const SolveDesc d, const InstPtrs ip) {
+    sep_body<NBP, MAXD, T>(d, ip, blockIdx.x);
+}
+template <int NBP, int MAXD, typename T>
+__global__ __launch_bounds__(512) void k_sep_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    sep_body<NBP, MAXD, T>(insts[t.x].d, insts[t.x].p, t.y);
+}
+
 template <int EPL, typename T>
 __global__ __launch_bounds__(256) void k_elem_per_lane(const SolveDesc d, const InstPtrs ip) {
     elem_body<EPL, T>(d, ip, blockIdx.x);
@@ -909,7 +1232,8 @@ struct DeviceSolve {
     size_t jpart_bytes = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
-    bool use_herm = true;  // paired Hermitian kernel when the blocks allow it
+    bool use_herm = true;   // paired Hermitian kernel when the blocks allow it
+    bool use_sep = true;    // row-separable kernel for Hermitian 3 <= nb <= 8
     float last_kernel_ms = 0.f;
 };
 
@@ -994,7 +1318,8 @@ static int pow2ceil(int n) {
 }
 
 // Which kernel and template instance a solve runs with.
-enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4 };
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6 };
+constexpr size_t LDS_MAX = 160 * 1024;
 struct LaunchKey {
     int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
     bool operator<(const LaunchKey& o) const {
@@ -1007,7 +1332,8 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
     const PairDesc& pd = ds->pd;
     LaunchKey k;
     k.fp32 = precision != 0;
-    if (pd.nrep > 0 && ds->use_herm && !(pd.vpaired && pd.nrep <= 4)) {
+    const bool reg_pairs = pd.vpaired && (pd.nx == 0 ? pd.nrep <= 4 : (pd.nrep <= 3 && pd.nx <= 2 && pd.nv <= 2));
+    if (pd.nrep > 0 && ds->use_herm && !reg_pairs) {
         k.kind = KIND_PAIRS;
         k.a = pd.nrep <= 12 ? 12 : 32;
         if (pd.nx == 0 && pd.nv <= 2) { k.b = 0; k.c = 2; }
@@ -1015,12 +1341,11 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
         else { k.b = 3; k.c = 3; }
         return k;
     }
-    if (pd.nrep > 0 && pd.nrep <= 4 && ds->use_herm) {
+    if (pd.nrep > 0 && reg_pairs && ds->use_herm) {
         k.kind = KIND_PAIR;
         k.a = pd.nrep <= 3 ? 3 : 4;
         if (pd.nx == 0) { k.b = 0; k.c = pd.nv <= 2 ? 2 : 3; }
-        else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
-        else { k.b = 3; k.c = 3; }
+        else { k.b = 2; k.c = 2; }
         return k;
     }
     if (d.nb <= 2 && d.ns <= 32) {
@@ -1030,6 +1355,20 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
         return k;
     }
     if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
+    if (d.nb >= 3 && d.nb <= 8 && d.hermhx && d.nd > 0 && ds->use_sep) {
+        // waves per workgroup: cover a k-row with equal chunks of at most 8 waves
+        const int nbp = d.nb <= 4 ? 4 : 8, gpw = 64 / ((nbp / 2) * (nbp / 2));
+        const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
+        int w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
+        while (w > 1 && sep_layout(d, nbp, k.fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
+        if (sep_layout(d, nbp, k.fp32 ? 4 : 8, w).bytes <= LDS_MAX) {
+            k.kind = KIND_SEP;
+            k.a = nbp;
+            k.b = w;
+            k.c = d.nd <= 5 ? 5 : 8;
+            return k;
+        }
+    }
     k.kind = KIND_ELEM;
     k.a = d.nb * d.nb <= 64 ? 1 : 4;
     return k;
@@ -1053,6 +1392,12 @@ static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
 
 // Tiles (workgroups) of one instance and waves per tile.
 static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block) {
+    if (k.kind == KIND_SEP) {
+        block = 64 * k.b;
+        waves_per_tile = 1;  // one current partial per workgroup and step
+        const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2)));
+        return d.N1 * ((d.N2 + kpt - 1) / kpt);
+    }
     if (k.kind == KIND_ELEM) {
         const int G = std::min(64, pow2ceil(d.nb * d.nb));
         block = 256;
@@ -1066,6 +1411,7 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 }
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
+    if (k.kind == KIND_SEP) return sep_layout(d, k.a, k.fp32 ? 4 : 8, k.b).bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = std::min(64, pow2ceil(d.nb * d.nb));
     const int kpb = 256 / G;
@@ -1086,8 +1432,7 @@ static void dispatch(const LaunchKey& k, F&& f) {
     if (k.a == R && k.b == X && k.c == V)                                                                \
         return go(k_pair2<R, X, V, double>, k_pair2_batch<R, X, V, double>, k_pair2<R, X, V, float>,     \
                   k_pair2_batch<R, X, V, float>);
-            PAIR(3, 0, 2) PAIR(4, 0, 2) PAIR(3, 0, 3) PAIR(4, 0, 3)
-            PAIR(3, 2, 2) PAIR(4, 2, 2) PAIR(3, 3, 3) PAIR(4, 3, 3)
+            PAIR(3, 0, 2) PAIR(4, 0, 2) PAIR(3, 0, 3) PAIR(4, 0, 3) PAIR(3, 2, 2)
 #undef PAIR
             break;
         case KIND_PAIRS:
@@ -1105,6 +1450,14 @@ static void dispatch(const LaunchKey& k, F&& f) {
                   k_thread_per_k_batch<NB, U, float>);
             THR(1, 16) THR(2, 8) THR(2, 16) THR(2, 32)
 #undef THR
+            break;
+        case KIND_SEP:
+#define SEP(NBP, D)                                                                                     \
+    if (k.a == NBP && k.c == D)                                                                          \
+        return go(k_sep<NBP, D, double>, k_sep_batch<NBP, D, double>, k_sep<NBP, D, float>,              \
+                  k_sep_batch<NBP, D, float>);
+            SEP(4, 5) SEP(4, 8) SEP(8, 5) SEP(8, 8)
+#undef SEP
             break;
         case KIND_ELEM:
             if (k.a == 1)
@@ -1137,7 +1490,7 @@ static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves) {
 }
 
 static void set_lds_limit(const void* fn, size_t lds) {
-    if (lds > 160 * 1024) throw std::runtime_error("LDS footprint exceeds 160 KiB");
+    if (lds > LDS_MAX) throw std::runtime_error("LDS footprint exceeds 160 KiB");
     if (lds > 64 * 1024) HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 }
 
@@ -1344,7 +1697,10 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
     return b->last_ms;
 }
 
-void dev_set_variant(DeviceSolve* ds, int variant) { ds->use_herm = (variant == 0); }
+void dev_set_variant(DeviceSolve* ds, int variant) {
+    ds->use_herm = (variant == 0);
+    ds->use_sep = (variant == 0);
+}
 
 int dev_kernel_variant(DeviceSolve* ds) { return classify(ds, 0).kind; }
 
@@ -1361,8 +1717,20 @@ double dev_flops_per_kstep(DeviceSolve* ds) {
         const double jcost = kind == KIND_PAIR ? pd.nv * (16.0 * pd.nrep + 7) : pd.nv * (32.0 * pd.nrep + 14);
         return 6.0 * pd.nrep + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 + jcost;
     }
-    int nv = 0;
-    for (int a = 0; a < 3; ++a) nv += (d.vmask >> a) & 1;
+    int nv = 0, nx = 0;
+    for (int a = 0; a < 3; ++a) { nv += (d.vmask >> a) & 1; nx += (d.xmask >> a) & 1; }
+    if (kind == KIND_SEP) {
+        // padded size P: stage sums 3 nd P^2 cMAC, products 4 P^3 cMAC, the
+        // M - M^dagger combination and RK4 axpys ~26 P^2, current nwd nv (4 cMAC
+        // + 1) per lane block; plus the per-row tables amortised over a row chunk
+        const LaunchKey k = classify(ds, 0);
+        const double P = k.a, P2 = P * P;
+        const double kpt = k.b * (64.0 / ((P / 2) * (P / 2)));
+        const double rows = (d.N2 + kpt - 1) / kpt * kpt;
+        const double table = d.ns * (6.0 + d.nb * d.nb * (3 * 8.0 + 6.0 * nx + 8.0 * nv));
+        return 8.0 * (3 * d.nd * P2 + 4 * P2 * P) + 26.0 * P2 + d.nwd * nv * (P2 / 4) * (4 * 8.0 + 8.0) +
+               table * (rows / d.N2) / kpt;
+    }
     const double nb = d.nb;
     return 8.0 * (8 * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;
 }

@@ -339,6 +339,33 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
         std::memcpy(&hs.blk[(size_t)s * NOPS * nb2 * 2], sup[s].b.data(), sizeof(cplx) * NOPS * nb2);
     }
 
+    // Supports grouped by n2 for the row-separable kernels.
+    d.hermhx = herm_hx ? 1 : 0;
+    {
+        std::vector<int> vals;
+        for (int s = 0; s < ns; ++s)
+            if (std::find(vals.begin(), vals.end(), d.n2[s]) == vals.end()) vals.push_back(d.n2[s]);
+        std::sort(vals.begin(), vals.end());
+        d.nd = 0;
+        d.nwd = 0;
+        if ((int)vals.size() <= SEP_MAXD) {
+            d.nd = (int)vals.size();
+            int q = 0;
+            for (int dd = 0; dd < d.nd; ++dd) {
+                d.dn2[dd] = vals[dd];
+                d.dseg[dd] = q;
+                bool hasv = false;
+                for (int s = 0; s < ns; ++s)
+                    if (d.n2[s] == vals[dd]) {
+                        d.sorder[q++] = s;
+                        hasv = hasv || (d.opmask[s] & (7u << OP_V));
+                    }
+                d.wdi[dd] = hasv ? d.nwd++ : -1;
+            }
+            d.dseg[d.nd] = q;
+        }
+    }
+
     // Per-step table: stage fields and the Peierls phase of every support point.
     std::vector<double> Rb(3 * (size_t)ns);
     for (int s = 0; s < ns; ++s)

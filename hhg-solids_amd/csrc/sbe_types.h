@@ -27,6 +27,12 @@ constexpr int STEP_HEAD = 12;     // doubles before the phases in a steptab row
 // and only the origin and one representative of each +-n pair are visited.
 enum { ROLE_ORIGIN = 0, ROLE_REP = 1, ROLE_PARTNER = 2 };
 
+// Row-separable sums: with k = m1 N2 + m2 and R_s = n1_s a1 + n2_s a2,
+//   sum_s B_s e^{-i(k-A).R_s} = sum_d tw2[m2 n2_d] G_d(m1, t),
+//   G_d = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} B_s,
+// so the supports are grouped by their distinct n2 values.
+constexpr int SEP_MAXD = 8;
+
 struct SolveDesc {
     int nb, N1, N2, N, nt, ns;
     int stride;                   // doubles per steptab row = STEP_HEAD + 2*ns
@@ -35,6 +41,13 @@ struct SolveDesc {
     int n1[MAX_SUPPORT], n2[MAX_SUPPORT];
     uint32_t opmask[MAX_SUPPORT]; // bit o set when block (s, o) is non-zero
     int role[MAX_SUPPORT];        // ROLE_* (meaningful when herm == 1)
+    int hermhx;                   // H and rbc blocks satisfy B(-n) = B(n)^dagger
+    int nd;                       // distinct n2 values of the supports (0 when > SEP_MAXD)
+    int dn2[SEP_MAXD];            // those values
+    int dseg[SEP_MAXD + 1];       // supports of value d: sorder[dseg[d] .. dseg[d + 1])
+    int sorder[MAX_SUPPORT];
+    int nwd;                      // values d with a velocity block; wdi[d] = their rank or -1
+    int wdi[SEP_MAXD];
 };
 
 // Pair pack of the Hermitian nb = 2 kernel. A +-R pair of blocks B, B^dagger

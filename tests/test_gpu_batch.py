@@ -40,7 +40,7 @@ def test_batch_members_equal_single_runs(tmp_path):
             alone.append(s.current()[1])
     solves = [lib.Solve(p) for p in params]
     kinds = sorted({s.kernel() for s in solves})
-    assert kinds == [2, 3, 4]  # register pairs, element-per-lane and streamed pairs in one batch
+    assert kinds == [2, 4, 6]  # register pairs, streamed pairs and row-separable kernels in one batch
     with lib.Batch(solves) as b:
         b.run()
     for p, s, Ja in zip(params, solves, alone):

@@ -104,20 +104,23 @@ def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax):
     assert np.abs(np.trace(rho, axis1=1, axis2=2) - tr0).max() < 1e-10
 
 
-@pytest.mark.parametrize("model", ["hmcase0", "haldane", "synthetic2"])
+@pytest.mark.parametrize("model", ["hmcase0", "haldane", "synthetic2", "synthetic4", "synthetic8"])
 def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model):
     """The Hermitian-paired nb=2 kernels (register pairs: real current;
     streamed pairs: complex current of an asymmetric window) and the generic
     thread-per-k kernel against the oracle on the same input."""
     from hhgamd import tbgen
-    if model == "synthetic2":
-        path = str(tmp_path / "syn2_tb.dat")
-        tbgen.write_tb(path, *tbgen.synthetic(2, seed=77))
-        expect = 4
+    if model.startswith("synthetic"):
+        nb = int(model[-1])
+        path = str(tmp_path / f"syn{nb}_tb.dat")
+        tbgen.write_tb(path, *tbgen.synthetic(nb, seed=77))
+        expect = {2: (4, 1), 4: (6, 3), 8: (6, 3)}[nb]  # streamed pairs / row-separable vs element-per-lane
     else:
         path = {"hmcase0": HMCASE0, "haldane": HALDANE}[model]
-        expect = 2
-    prm = make_param(path, nr=(16, 20), tmax=30.0, tmax_field=25.0, intensity=3e12, pol=(0.6, 0.8, 0.0))
+        expect = (2, 1)
+    nr = (16, 20) if "8" not in model else (8, 10)
+    prm = make_param(path, nr=nr, tmax=30.0 if "8" not in model else 8.0, tmax_field=25.0 if "8" not in model else 7.0,
+                     intensity=3e12, pol=(0.6, 0.8, 0.0))
     _, Jref = oracle_lib.run(path, prm, mode="perk")
     out = {}
     for kern in ("auto", "generic"):
@@ -128,7 +131,7 @@ def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model
         err = rel_l2(J, Jref)
         print(f"{model} {kern} kernel {out[kern]}: rel L2 vs oracle {err:.2e}")
         assert err <= FP64_TOL, kern
-    assert out == {"auto": expect, "generic": 1}
+    assert out == {"auto": expect[0], "generic": expect[1]}
     if model == "synthetic2":
         assert np.abs(Jref.imag).max() > 1e-6 * np.abs(Jref).max()  # the window current is complex here
 
