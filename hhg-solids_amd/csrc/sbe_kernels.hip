@@ -18,7 +18,7 @@
 //   k_pair2s<NREP,NX,NV,T>     nb = 2 with Hermitian H and rbc, pair
 //                              coefficients streamed through scalar loads;
 //                              complex (window) current.
-//   k_sep<NBP,MAXD,T>          Hermitian 3 <= nb <= 8: row-separable Fourier
+//   k_sep<NBP,MAXD,T>          Hermitian 3 <= nb <= 16: row-separable Fourier
 //                              sums built once per k-row and step in LDS,
 //                              2x2 element blocks per lane, M - M^dagger
 //                              commutator.
@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <stdexcept>
@@ -814,34 +815,38 @@ __device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // --------------------------------------------------------------------------
-// Row-separable element kernel: Hermitian H and rbc, 3 <= nb <= NBP (4 or 8,
-// smaller nb zero-padded). A workgroup of W waves owns a chunk of one k-row
+// Row-separable element kernel: Hermitian H and rbc, 3 <= nb <= NBP (4, 8 or
+// 16, smaller nb zero-padded). A workgroup of W waves owns a chunk of one k-row
 // m1. Per step it builds, cooperatively and double-buffered in LDS,
 //   G[j][d]  = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} (H_s + E_j . rbc_s)   (stage fields j = 0..2)
-//   GV[w][v] = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} v_s                  (values d with a v block)
+//   GV[d][a] = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} v_a,s                 (axes a = x, y, z)
 // and every k-point forms its stage Hamiltonian sum_d tw2[m2 n2_d] G[j][d]
-// with nd complex MACs per element instead of ns. For Hermitian H and X the
+// with MAXD complex MACs per element instead of ns. For Hermitian H and X the
 // commutator is -i(M - M^dagger) with M = H X: one complex product per stage.
 // Each lane owns a 2x2 element block, so every LDS read of the product feeds
 // two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and exchange H,
 // X and M through LDS with wave-level ordering only. One workgroup barrier per
-// step (the G tables of step t+1 are built while step t runs).
+// step: the tables of step t+1 are built while step t runs. The table build
+// loads every operator block of a support unconditionally (absent blocks are
+// zero in HBM), so its loads issue together instead of one per branch.
 struct SepLayout {
-    size_t off_g, gbytes, off_gv, gvbytes, off_k, kbytes, off_jw, bytes;
-    int nv;
+    size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_k, kbytes, off_jw, bytes;
 };
-__host__ __device__ inline SepLayout sep_layout(const SolveDesc& d, int NBP, int tsize, int W) {
+// Tables of NBP <= 8 are double-buffered (built one step ahead); NBP = 16
+// keeps one buffer and builds between two barriers.
+__host__ __device__ constexpr int sep_nbuf(int NBP) { return NBP <= 8 ? 2 : 1; }
+__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W) {
     SepLayout L;
-    L.nv = 0;
-    for (int a = 0; a < 3; ++a) L.nv += (d.vmask >> a) & 1;
     const size_t E = (size_t)NBP * NBP;
     const int GPW = 64 / ((NBP / 2) * (NBP / 2));
+    const int nbuf = sep_nbuf(NBP);
     size_t o = 0;
-    L.gbytes = 3 * (size_t)d.nd * E * 2 * tsize;
-    L.off_g = o; o = align16(o + 2 * L.gbytes);
-    L.gvbytes = (size_t)d.nwd * L.nv * E * 16;
-    L.off_gv = o; o = align16(o + 2 * L.gvbytes);
-    L.kbytes = 2 * E * 2 * tsize + 16;  // H/M and X of one k-point; the pad staggers banks
+    L.gbytes = 3 * (size_t)MAXD * E * 2 * tsize;
+    L.off_g = o; o = align16(o + nbuf * L.gbytes);
+    L.gvbytes = (size_t)MAXD * 3 * E * 16;
+    L.off_gv = o; o = align16(o + nbuf * L.gvbytes);
+    L.off_sup = o; o = align16(o + MAX_SUPPORT * 16 + MAX_SUPPORT * 4 + 16 * 4 + (size_t)MAXD * E * 4);
+    L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
     L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
     L.off_jw = o; o = align16(o + sizeof(double) * 2 * W * 6);
     L.bytes = o;
@@ -855,25 +860,42 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// Two consecutive complex values (16-byte aligned pairs).
-__device__ __forceinline__ void ld2(const cx<float>* p, cx<float>& a, cx<float>& b) {
-    const float4 v = *reinterpret_cast<const float4*>(p);
-    a = {v.x, v.y};
-    b = {v.z, v.w};
-}
-__device__ __forceinline__ void ld2(const cx<double>* p, cx<double>& a, cx<double>& b) {
-    const double2 v0 = reinterpret_cast<const double2*>(p)[0];
-    const double2 v1 = reinterpret_cast<const double2*>(p)[1];
-    a = {v0.x, v0.y};
-    b = {v1.x, v1.y};
-}
-__device__ __forceinline__ void st2(cx<float>* p, cx<float> a, cx<float> b) {
-    *reinterpret_cast<float4*>(p) = make_float4(a.re, a.im, b.re, b.im);
-}
-__device__ __forceinline__ void st2(cx<double>* p, cx<double> a, cx<double> b) {
-    reinterpret_cast<double2*>(p)[0] = make_double2(a.re, a.im);
-    reinterpret_cast<double2*>(p)[1] = make_double2(b.re, b.im);
-}
+// Element placement of an NBP x NBP tile in LDS, chosen so that the 16 lanes
+// of a k-point (2x2 blocks (bi, bj)) touch one contiguous 256-byte span in
+// every access: block-row writes, row pairs (m, i0..i0+1) of the product and
+// the transposed pairs. FP32: planes by row parity, then block, then the
+// column pair (16 bytes); FP64: planes by (row, column) parity, then block.
+template <typename T, int NBP>
+struct Tile;
+template <int NBP>
+struct Tile<float, NBP> {
+    static constexpr int HB = NBP / 2, B2 = HB * HB;
+    __host__ __device__ static int off(int i, int j) { return (((i & 1) * B2 + (i >> 1) * HB + (j >> 1)) << 1) + (j & 1); }
+    __device__ static void ld2(const cx<float>* t, int i, int j0, cx<float>& a, cx<float>& b) {
+        const float4 v = *reinterpret_cast<const float4*>(t + off(i, j0));
+        a = {v.x, v.y};
+        b = {v.z, v.w};
+    }
+    __device__ static void st2(cx<float>* t, int i, int j0, cx<float> a, cx<float> b) {
+        *reinterpret_cast<float4*>(t + off(i, j0)) = make_float4(a.re, a.im, b.re, b.im);
+    }
+};
+template <int NBP>
+struct Tile<double, NBP> {
+    static constexpr int HB = NBP / 2, B2 = HB * HB;
+    __host__ __device__ static int off(int i, int j) { return (((i & 1) << 1) + (j & 1)) * B2 + (i >> 1) * HB + (j >> 1); }
+    __device__ static void ld2(const cx<double>* t, int i, int j0, cx<double>& a, cx<double>& b) {
+        const double2 v0 = *reinterpret_cast<const double2*>(t + off(i, j0));
+        const double2 v1 = *reinterpret_cast<const double2*>(t + off(i, j0) + B2);
+        a = {v0.x, v0.y};
+        b = {v1.x, v1.y};
+    }
+    __device__ static void st2(cx<double>* t, int i, int j0, cx<double> a, cx<double> b) {
+        *reinterpret_cast<double2*>(t + off(i, j0)) = make_double2(a.re, a.im);
+        *reinterpret_cast<double2*>(t + off(i, j0) + B2) = make_double2(b.re, b.im);
+    }
+};
+
 template <typename T>
 __device__ __forceinline__ void cfma(cx<T>& acc, cx<T> a, cx<T> b) {  // acc += a*b
     acc.re = fmaT(a.re, b.re, fmaT(-a.im, b.im, acc.re));
@@ -885,13 +907,40 @@ __device__ __forceinline__ void cfmac(cx<T>& acc, cx<T> a, cx<T> b) {  // acc +=
     acc.im = fmaT(a.re, b.im, fmaT(-a.im, b.re, acc.im));
 }
 
+// Sum over the 16 lanes of each DPP row, returned in every lane of the row
+// (quad swaps, half-row mirror, row mirror); no LDS traffic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row_sum16(double v) {
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    v += dpp_mov<0x140>(v);  // row_mirror
+    return v;
+}
+// Wave total, valid in every lane: row sums combined through readlane.
+__device__ __forceinline__ double wave_total(double v) {
+    v = row_sum16(v);
+    auto rl = [&](int l) {
+        return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                                __builtin_amdgcn_readlane(__double2loint(v), l));
+    };
+    return (rl(0) + rl(16)) + (rl(32) + rl(48));
+}
+
 template <int NBP, int MAXD, typename T>
 __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip, int tile) {
     constexpr int HB = NBP / 2, LPK = HB * HB, GPW = 64 / LPK, E = NBP * NBP;
+    using TT = Tile<T, NBP>;
+    using TD = Tile<double, NBP>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int W = blockDim.x >> 6;
-    const SepLayout L = sep_layout(d, NBP, (int)sizeof(T), W);
-    const int nb = d.nb, nbe = nb * nb, nd = d.nd, nv = L.nv, N = d.N, N1 = d.N1, N2 = d.N2;
+    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W);
+    const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / LPK, bl = lane - g * LPK;
@@ -902,65 +951,69 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int m2 = (tile - m1 * tpr) * kpt + wave * GPW + g;
     const bool kval = m2 < N2;
     const int k = m1 * N2 + (kval ? m2 : 0);
-    int xax[3] = {0, 0, 0}, vax[3] = {0, 0, 0}, nx = 0;
-    {
-        int c = 0;
-        for (int a = 0; a < 3; ++a) {
-            if ((d.xmask >> a) & 1) xax[nx++] = a;
-            if ((d.vmask >> a) & 1) vax[c++] = a;
-        }
-    }
     cx<T>* bufH = reinterpret_cast<cx<T>*>(smem + L.off_k + (size_t)(wave * GPW + g) * L.kbytes);  // H, then M
     cx<T>* bufX = bufH + E;
     double* jw = reinterpret_cast<double*>(smem + L.off_jw);
+    // per-support table in n2-sorted order: row twiddle tw1[m1 n1_s], index s;
+    // segment bounds of each value d; entry codes (d << 16 | el << 8 | i << 4 | j)
+    cx<double>* sw1 = reinterpret_cast<cx<double>*>(smem + L.off_sup);
+    int* ss = reinterpret_cast<int*>(smem + L.off_sup + MAX_SUPPORT * 16);
+    int* seg = ss + MAX_SUPPORT;
+    int* ent = seg + 16;
+    const int nent = nd * nbe;
 
-    // zero the tables once: padded elements are never written
-    for (size_t i = tid; i < L.off_k / 16; i += bd) reinterpret_cast<double2*>(smem)[i] = make_double2(0.0, 0.0);
+    for (size_t i = tid; i < (L.off_sup - L.off_g) / 16; i += bd)
+        reinterpret_cast<double2*>(smem + L.off_g)[i] = make_double2(0.0, 0.0);
+    for (int q = tid; q < d.ns; q += bd) {
+        const int s = d.sorder[q];
+        ss[q] = s;
+        sw1[q] = ld<double>(ip.tw1 + 2 * posmod((long)m1 * d.n1[s], N1));
+    }
+    for (int q = tid; q <= nd; q += bd) seg[q] = d.dseg[q];
+    for (int q = tid; q < nent; q += bd) {
+        const int dd = q / nbe, el = q - dd * nbe;
+        ent[q] = (dd << 16) | (el << 8) | ((el / nb) << 4) | (el - (el / nb) * nb);  // el < 256, i, j < 16
+    }
     __syncthreads();
 
     auto build = [&](int ti, int buf) {
         const double* st = ip.steptab + (size_t)ti * d.stride;
+        double ef[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ef[c] = st[c];
         cx<T>* G = reinterpret_cast<cx<T>*>(smem + L.off_g + buf * L.gbytes);
         cx<double>* GV = reinterpret_cast<cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
-        for (int q = tid; q < nd * nbe; q += bd) {
-            const int dd = q / nbe, el = q - dd * nbe;
-            const int ep = (el / nb) * NBP + (el - (el / nb) * nb);
-            cx<double> a0 = {0, 0}, a1 = {0, 0}, a2 = {0, 0};
-            cx<double> av[3] = {{0, 0}, {0, 0}, {0, 0}};
-            for (int qq = d.dseg[dd]; qq < d.dseg[dd + 1]; ++qq) {
-                const int s = d.sorder[qq];
-                const uint32_t om = d.opmask[s];
-                const cx<double> w = cmul(ld<double>(ip.tw1 + 2 * posmod((long)m1 * d.n1[s], N1)),
-                                          ld<double>(st + STEP_HEAD + 2 * s));
+        for (int q = tid; q < nent; q += bd) {
+            const int code = ent[q];
+            const int dd = code >> 16, el = (code >> 8) & 255, ei = (code >> 4) & 15, ej = code & 15;
+            // sum_s w_s B_s for H, the three rbc axes and the three v axes
+            cx<double> a[NOPS];
+#pragma unroll
+            for (int o = 0; o < NOPS; ++o) a[o] = {0, 0};
+            const int qe = seg[dd + 1];
+            for (int qq = seg[dd]; qq < qe; ++qq) {
+                const int s = ss[qq];
                 const double* bs = ip.blk + 2 * ((size_t)s * NOPS * nbe + el);
-                const cx<double> h = (om & (1u << OP_H)) ? ld<double>(bs) : cx<double>{0, 0};
-                cx<double> b0 = h, b1 = h, b2 = h;
+                cx<double> ob[NOPS];
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
+                const cx<double> w = cmul(sw1[qq], ld<double>(st + STEP_HEAD + 2 * s));
+#pragma unroll
+                for (int o = 0; o < NOPS; ++o) cfma(a[o], w, ob[o]);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                cx<double> b = a[OP_H];
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
-                    if (x >= nx || !(om & (1u << (OP_X + xax[x])))) continue;
-                    const int ax = xax[x];
-                    const cx<double> X = ld<double>(bs + 2 * (size_t)(OP_X + ax) * nbe);
-                    b0.re = fmaT(st[ax], X.re, b0.re); b0.im = fmaT(st[ax], X.im, b0.im);
-                    b1.re = fmaT(st[3 + ax], X.re, b1.re); b1.im = fmaT(st[3 + ax], X.im, b1.im);
-                    b2.re = fmaT(st[6 + ax], X.re, b2.re); b2.im = fmaT(st[6 + ax], X.im, b2.im);
+                    b.re = fmaT(ef[3 * j + x], a[OP_X + x].re, b.re);
+                    b.im = fmaT(ef[3 * j + x], a[OP_X + x].im, b.im);
                 }
-                cfma(a0, w, b0);
-                cfma(a1, w, b1);
-                cfma(a2, w, b2);
-#pragma unroll
-                for (int v = 0; v < 3; ++v)
-                    if (v < nv && (om & (1u << (OP_V + vax[v]))))
-                        cfma(av[v], w, ld<double>(bs + 2 * (size_t)(OP_V + vax[v]) * nbe));
+                G[(size_t)(j * MAXD + dd) * E + TT::off(ei, ej)] = cast<T>(b);
             }
-            G[(size_t)(0 * nd + dd) * E + ep] = cast<T>(a0);
-            G[(size_t)(1 * nd + dd) * E + ep] = cast<T>(a1);
-            G[(size_t)(2 * nd + dd) * E + ep] = cast<T>(a2);
-            const int wv = d.wdi[dd];
-            if (wv >= 0) {
+            cx<double>* av = a + OP_V;
 #pragma unroll
-                for (int v = 0; v < 3; ++v)
-                    if (v < nv) GV[((size_t)wv * nv + v) * E + ep] = av[v];
-            }
+            for (int x = 0; x < 3; ++x) GV[(size_t)(dd * 3 + x) * E + TD::off(ei, ej)] = av[x];
         }
     };
 
@@ -976,14 +1029,19 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     for (int dd = 0; dd < MAXD; ++dd)
         z[dd] = (dd < nd && kval) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2)) : cx<double>{0.0, 0.0};
 
+    int wmask = 0;
+#pragma unroll
+    for (int dd = 0; dd < MAXD; ++dd)
+        if (dd < nd && d.wdi[dd] >= 0) wmask |= 1 << dd;
     const T half = (T)(0.5 * ip.dt);
     const T full = (T)ip.dt;
     const double pref = 0.1666666666666 * ip.dt;
     build(0, 0);
     __syncthreads();
+    constexpr bool DB = sep_nbuf(NBP) == 2;
     for (int ti = 0; ti < d.nt; ++ti) {
-        const int buf = ti & 1;
-        if (ti + 1 < d.nt) build(ti + 1, buf ^ 1);
+        const int buf = DB ? (ti & 1) : 0, jb = ti & 1;
+        if (DB && ti + 1 < d.nt) build(ti + 1, buf ^ 1);
         const cx<T>* G = reinterpret_cast<const cx<T>*>(smem + L.off_g + buf * L.gbytes);
         const cx<double>* GV = reinterpret_cast<const cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
         cx<T> Hs[4], rs[4], K[4], acc[4];
@@ -992,12 +1050,11 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             for (int q = 0; q < 4; ++q) Hs[q] = {(T)0, (T)0};
 #pragma unroll
             for (int dd = 0; dd < MAXD; ++dd) {
-                if (dd >= nd) continue;
                 const cx<T> zz = cast<T>(z[dd]);
-                const cx<T>* gp = G + (size_t)(j * nd + dd) * E + i0 * NBP + j0;
+                const cx<T>* gp = G + (size_t)(j * MAXD + dd) * E;
                 cx<T> g0, g1, g2, g3;
-                ld2(gp, g0, g1);
-                ld2(gp + NBP, g2, g3);
+                TT::ld2(gp, i0, j0, g0, g1);
+                TT::ld2(gp, i0 + 1, j0, g2, g3);
                 cfma(Hs[0], zz, g0);
                 cfma(Hs[1], zz, g1);
                 cfma(Hs[2], zz, g2);
@@ -1005,10 +1062,10 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             }
         };
         auto deriv = [&]() {
-            st2(bufH + i0 * NBP + j0, Hs[0], Hs[1]);
-            st2(bufH + (i0 + 1) * NBP + j0, Hs[2], Hs[3]);
-            st2(bufX + i0 * NBP + j0, rs[0], rs[1]);
-            st2(bufX + (i0 + 1) * NBP + j0, rs[2], rs[3]);
+            TT::st2(bufH, i0, j0, Hs[0], Hs[1]);
+            TT::st2(bufH, i0 + 1, j0, Hs[2], Hs[3]);
+            TT::st2(bufX, i0, j0, rs[0], rs[1]);
+            TT::st2(bufX, i0 + 1, j0, rs[2], rs[3]);
             wave_lds_sync();
             cx<T> M[4];
 #pragma unroll
@@ -1016,20 +1073,20 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
             for (int m = 0; m < NBP; ++m) {
                 cx<T> h0, h1, r0, r1;
-                ld2(bufH + m * NBP + i0, h0, h1);  // H[m][i0..i0+1] = conj(H[i0..i0+1][m])
-                ld2(bufX + m * NBP + j0, r0, r1);  // X[m][j0..j0+1]
+                TT::ld2(bufH, m, i0, h0, h1);  // H[m][i0..i0+1] = conj(H[i0..i0+1][m])
+                TT::ld2(bufX, m, j0, r0, r1);  // X[m][j0..j0+1]
                 cfmac(M[0], h0, r0);
                 cfmac(M[1], h0, r1);
                 cfmac(M[2], h1, r0);
                 cfmac(M[3], h1, r1);
             }
             wave_lds_sync();
-            st2(bufH + i0 * NBP + j0, M[0], M[1]);
-            st2(bufH + (i0 + 1) * NBP + j0, M[2], M[3]);
+            TT::st2(bufH, i0, j0, M[0], M[1]);
+            TT::st2(bufH, i0 + 1, j0, M[2], M[3]);
             wave_lds_sync();
             cx<T> t0, t1, u0, u1;
-            ld2(bufH + j0 * NBP + i0, t0, t1);        // M[j0][i0], M[j0][i0+1]
-            ld2(bufH + (j0 + 1) * NBP + i0, u0, u1);  // M[j0+1][i0], M[j0+1][i0+1]
+            TT::ld2(bufH, j0, i0, t0, t1);      // M[j0][i0], M[j0][i0+1]
+            TT::ld2(bufH, j0 + 1, i0, u0, u1);  // M[j0+1][i0], M[j0+1][i0+1]
             // K_ij = -i (M_ij - conj(M_ji))
             K[0] = {M[0].im + t0.im, t0.re - M[0].re};
             K[1] = {M[1].im + u0.im, u0.re - M[1].re};
@@ -1067,44 +1124,42 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             rho[q].re = fmaT(pref, (double)acc[q].re, rho[q].re);
             rho[q].im = fmaT(pref, (double)acc[q].im, rho[q].im);
         }
-        // current of this step: sum_d tw2[m2 n2_d] sum_ij GV[d][ij] conj(rho_ij)
+        // current of this step: sum_d tw2[m2 n2_d] sum_ij GV[d][a][ij] conj(rho_ij)
         cx<double> jv[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
         for (int dd = 0; dd < MAXD; ++dd) {
-            const int wv = dd < nd ? d.wdi[dd] : -1;
-            if (wv < 0) continue;
+            if (!((wmask >> dd) & 1)) continue;  // no velocity block at this n2 (uniform)
 #pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                if (v >= nv) continue;
-                const cx<double>* vp = GV + ((size_t)wv * nv + v) * E + i0 * NBP + j0;
+            for (int a = 0; a < 3; ++a) {
+                if (!((d.vmask >> a) & 1)) continue;
+                const cx<double>* vp = GV + (size_t)(dd * 3 + a) * E;
                 cx<double> v0, v1, v2, v3;
-                ld2(vp, v0, v1);
-                ld2(vp + NBP, v2, v3);
+                TD::ld2(vp, i0, j0, v0, v1);
+                TD::ld2(vp, i0 + 1, j0, v2, v3);
                 cx<double> s = {0, 0};
                 cfmac(s, rho[0], v0);
                 cfmac(s, rho[1], v1);
                 cfmac(s, rho[2], v2);
                 cfmac(s, rho[3], v3);
-                cfma(jv[v], z[dd], s);
+                cfma(jv[a], z[dd], s);
             }
         }
-        double jc[6] = {0, 0, 0, 0, 0, 0};
+        double jc[6] = {jv[0].re, jv[0].im, jv[1].re, jv[1].im, jv[2].re, jv[2].im};
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int v = 0; v < 3; ++v)
-                if (v < nv && vax[v] == a) { jc[2 * a] = jv[v].re; jc[2 * a + 1] = jv[v].im; }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) jc[c] = wave_sum(jc[c]);
+        for (int c = 0; c < 6; ++c) jc[c] = wave_total(jc[c]);
         if (lane == 0) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) jw[(buf * W + wave) * 6 + c] = jc[c];
+            for (int c = 0; c < 6; ++c) jw[(jb * W + wave) * 6 + c] = jc[c];
         }
         __syncthreads();
         if (tid < 6) {
             double sacc = 0.0;
-            for (int w = 0; w < W; ++w) sacc += jw[(buf * W + w) * 6 + tid];
+            for (int w = 0; w < W; ++w) sacc += jw[(jb * W + w) * 6 + tid];
             ip.jpart[((size_t)ti * ip.nwaves + tile) * 6 + tid] = sacc;
+        }
+        if (!DB && ti + 1 < d.nt) {
+            build(ti + 1, 0);
+            __syncthreads();
         }
     }
 #pragma unroll
@@ -1233,7 +1288,7 @@ struct DeviceSolve {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     bool use_herm = true;   // paired Hermitian kernel when the blocks allow it
-    bool use_sep = true;    // row-separable kernel for Hermitian 3 <= nb <= 8
+    bool use_sep = true;    // row-separable kernel for Hermitian 3 <= nb <= 16
     float last_kernel_ms = 0.f;
 };
 
@@ -1355,13 +1410,13 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
         return k;
     }
     if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
-    if (d.nb >= 3 && d.nb <= 8 && d.hermhx && d.nd > 0 && ds->use_sep) {
+    if (d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep) {
         // waves per workgroup: cover a k-row with equal chunks of at most 8 waves
-        const int nbp = d.nb <= 4 ? 4 : 8, gpw = 64 / ((nbp / 2) * (nbp / 2));
+        const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
         const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
         int w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
-        while (w > 1 && sep_layout(d, nbp, k.fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
-        if (sep_layout(d, nbp, k.fp32 ? 4 : 8, w).bytes <= LDS_MAX) {
+        while (w > 1 && sep_layout(nbp, d.nd <= 5 ? 5 : 8, k.fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
+        if (sep_layout(nbp, d.nd <= 5 ? 5 : 8, k.fp32 ? 4 : 8, w).bytes <= LDS_MAX) {
             k.kind = KIND_SEP;
             k.a = nbp;
             k.b = w;
@@ -1411,7 +1466,7 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 }
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
-    if (k.kind == KIND_SEP) return sep_layout(d, k.a, k.fp32 ? 4 : 8, k.b).bytes;
+    if (k.kind == KIND_SEP) return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b).bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = std::min(64, pow2ceil(d.nb * d.nb));
     const int kpb = 256 / G;
@@ -1456,7 +1511,7 @@ static void dispatch(const LaunchKey& k, F&& f) {
     if (k.a == NBP && k.c == D)                                                                          \
         return go(k_sep<NBP, D, double>, k_sep_batch<NBP, D, double>, k_sep<NBP, D, float>,              \
                   k_sep_batch<NBP, D, float>);
-            SEP(4, 5) SEP(4, 8) SEP(8, 5) SEP(8, 8)
+            SEP(4, 5) SEP(4, 8) SEP(8, 5) SEP(8, 8) SEP(16, 5) SEP(16, 8)
 #undef SEP
             break;
         case KIND_ELEM:
@@ -1720,16 +1775,17 @@ double dev_flops_per_kstep(DeviceSolve* ds) {
     int nv = 0, nx = 0;
     for (int a = 0; a < 3; ++a) { nv += (d.vmask >> a) & 1; nx += (d.xmask >> a) & 1; }
     if (kind == KIND_SEP) {
-        // padded size P: stage sums 3 nd P^2 cMAC, products 4 P^3 cMAC, the
-        // M - M^dagger combination and RK4 axpys ~26 P^2, current nwd nv (4 cMAC
-        // + 1) per lane block; plus the per-row tables amortised over a row chunk
+        // padded size P, table depth D (MAXD): stage sums 3 D P^2 cMAC,
+        // products 4 P^3 cMAC, the M - M^dagger combination and RK4 axpys
+        // ~26 P^2 flops, current 3 D (4 + 1) cMAC per 2x2 lane block; plus the
+        // per-row tables (90 flops per support and element) amortised over
+        // the k-points of a workgroup
         const LaunchKey k = classify(ds, 0);
-        const double P = k.a, P2 = P * P;
+        const double P = k.a, P2 = P * P, D = k.c;
         const double kpt = k.b * (64.0 / ((P / 2) * (P / 2)));
-        const double rows = (d.N2 + kpt - 1) / kpt * kpt;
-        const double table = d.ns * (6.0 + d.nb * d.nb * (3 * 8.0 + 6.0 * nx + 8.0 * nv));
-        return 8.0 * (3 * d.nd * P2 + 4 * P2 * P) + 26.0 * P2 + d.nwd * nv * (P2 / 4) * (4 * 8.0 + 8.0) +
-               table * (rows / d.N2) / kpt;
+        const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);
+        const double table = 90.0 * d.ns * d.nb * d.nb;
+        return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + 3 * D * (P2 / 4) * 40.0 + table / (kpt * used);
     }
     const double nb = d.nb;
     return 8.0 * (8 * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;

@@ -51,7 +51,7 @@ def test_fp32_hmcase0_readme():
     assert err <= J_TOL and worst <= HARMONIC_TOL
 
 
-@pytest.mark.parametrize("nb", [2, 4, 8])
+@pytest.mark.parametrize("nb", [2, 4, 8, 16])
 def test_fp32_synthetic(tmp_path, nb):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
