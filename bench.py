@@ -34,6 +34,24 @@ METRIC = "k-point·RK4-steps/s; p50/p99 solve latency; J(t) rel-L2 err vs FP64"
 UNIT = "k-point·RK4-steps/s"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 PEAK_HBM_GBS = 8000.0
+KERNELS = {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane", 4: "k_pair2s", 6: "k_sep"}
+
+
+def j_errors(t, J, Jref, lam):
+    """J(t) relative L2 over the 3 x nt complex array and the worst relative
+    deviation of the per-harmonic intensities up to the cutoff (the highest
+    odd harmonic within 1e-6 of the strongest), per axis with a current."""
+    from hhgamd import spectrum
+    err = float(np.linalg.norm(J - Jref) / np.linalg.norm(Jref))
+    worst = 0.0
+    for a in range(3):
+        if np.abs(Jref[:, a]).max() == 0:
+            continue
+        q, I = spectrum.harmonic_intensities(t, Jref[:, a], lam)
+        _, I2 = spectrum.harmonic_intensities(t, J[:, a], lam)
+        sel = q <= spectrum.cutoff_order(q, I)
+        worst = max(worst, float(np.max(np.abs(I2[sel] / I[sel] - 1.0))))
+    return err, worst
 
 
 def workload_param(path_tb, nr, precision):
@@ -91,9 +109,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size oracle J check")
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"],
-                    help="plain: the FP64 200x200 solve (default); batch: 64 mixed instances in one batched run; "
-                         "serve: Poisson request streams through the continuous-batching server")
+    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "batch", "serve"],
+                    help="plain: the FP64 200x200 solve (default); precision: the 8-band 120x120 solve with FP32 "
+                         "stages against FP64; batch: 64 mixed instances in one batched run; serve: Poisson request "
+                         "streams through the continuous-batching server")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--rates", default="0.5,2,5,20", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=16, help="serve: requests per rate")
@@ -107,6 +126,8 @@ def main():
 
     if args.mode == "batch":
         return bench_batch(args)
+    if args.mode == "precision":
+        return bench_precision(args)
     if args.mode == "serve":
         return bench_serve(args)
 
@@ -151,7 +172,7 @@ def main():
         "roofline": {"bound": "mfma", "alu": "FP64 VALU (v_fma_f64): MI355X FP64 vector peak = FP64 matrix peak",
                      "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
-                     "kernel_ms": kms, "kernel": {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane"}.get(kernel_id),
+                     "kernel_ms": kms, "kernel": KERNELS.get(kernel_id),
                      "flops_per_kstep": F, "generic_flops_per_kstep": F_generic,
                      "generic_equivalent_tflops": F_generic * nk * nt / (kms * 1e-3) / 1e12,
                      "staged_state_GBs": 2 * 16 * nb * nb * nk * nt / (kms * 1e-3) / 1e9},
@@ -172,10 +193,62 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def bench_precision(args):
+    """Seeded 8-band Wannier model (5x5 R shell), 120x120 k-grid, I = 5e11,
+    readme pulse otherwise: the mixed path (FP32 commutator and RK4 stages,
+    FP64 state, initial state and current) timed, its J(t) compared with the
+    FP64 path of the same solve."""
+    from hhgamd import lib, workloads
+    tmp = tempfile.mkdtemp(prefix="hhg_bench_")
+    prm = workloads.c3(tmp, precision=args.precision if args.precision != "fp64" else "fp32")
+    s = lib.Solve(prm)
+    nk, nt, nb = s.N, s.nt, s.nb
+    F = s.flops_per_kstep()
+    for _ in range(args.warmup):
+        s.run()
+    lat, kern = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        s.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+        kern.append(s.kernel_ms())
+    wall = time.perf_counter() - t0
+    t, J32 = s.current()
+    kernel_id = s.kernel()
+    s.close()
+    with lib.Solve(dict(prm, precision="fp64")) as s64:
+        s64.run()
+        t64 = time.perf_counter()
+        s64.run()
+        fp64_s = time.perf_counter() - t64
+        _, J64 = s64.current()
+        fp64_kms = s64.kernel_ms()
+    err, worst = j_errors(t, J32, J64, prm["lambda"])
+    kms = float(np.mean(kern))
+    achieved = F * nk * nt / (kms * 1e-3) / 1e12
+    out = {"metric": METRIC, "value": nk * nt * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"synthetic-8band-{prm['nr1']}x{prm['nr2']}-nt{nt}-fp32", "nk": nk, "nb": nb,
+                      "nt": nt, "precision": "fp32 stages, fp64 state/current"},
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "j_rel_l2_vs_fp64": err, "worst_harmonic_dev_vs_fp64": worst,
+           "fp64_value": nk * nt / fp64_s, "fp64_kernel_ms": fp64_kms,
+           "roofline": {"bound": "mfma", "alu": "FP32 VALU without packed math (78.6 TF attainable), FP64 current",
+                        "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None, "kernel_ms": kms,
+                        "kernel": KERNELS.get(kernel_id), "flops_per_kstep": F,
+                        "generic_flops_per_kstep": algorithmic_flops_per_kstep(nb, 25)}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args):
     """64 seeded instances (pol angle, intensity, wavelength, nb in {2,4,8},
-    uneven 5-smooth k-grids) propagated by one batched run per step; the same
-    instances run one by one are timed for comparison."""
+    uneven 5-smooth k-grids) propagated by one batched run per step. Also
+    timed: the same instances one by one in the same precision, and the plain
+    FP64 path one by one (the reference point of the mixed/batched form);
+    every member's J(t) is compared with its FP64 solve."""
     from hhgamd import lib, workloads
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
     params = workloads.c4(tmp, n=args.instances, precision=args.precision)
@@ -201,16 +274,35 @@ def bench_batch(args):
     for s, p in zip(solves, params):
         kinds.setdefault(f"nb{s.nb}/k{s.kernel()}", 0)
         kinds[f"nb{s.nb}/k{s.kernel()}"] += 1
+    # plain FP64 path, one solve at a time, same instances; J(t) of each for the error check
+    errs, worsts = [], []
+    plain_s = 0.0
+    for s, p in zip(solves, params):
+        t, Jb = s.current()
+        with lib.Solve(dict(p, precision="fp64")) as s64:
+            a = time.perf_counter()
+            s64.run()
+            plain_s += time.perf_counter() - a
+            _, J64 = s64.current()
+        if args.precision != "fp64":
+            e, w = j_errors(t, Jb, J64, p["lambda"])
+            errs.append(e)
+            worsts.append(w)
     for s in solves:
         s.close()
-    out = {"metric": METRIC, "value": total * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+    value = total * args.steps / wall
+    out = {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
            "data": "synthetic",
            "config": {"workload": f"batch-{len(params)}-mixed-nb248-{args.precision}", "instances": len(params),
                       "kpoint_steps_per_pass": total, "kernels": kinds},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-           "one_by_one_value": total / seq, "batch_speedup_vs_one_by_one": seq / (wall / args.steps)}
+           "one_by_one_value": total / seq, "batch_speedup_vs_one_by_one": seq / (wall / args.steps),
+           "plain_fp64_one_by_one_value": total / plain_s, "vs_plain_fp64": value * plain_s / total}
+    if errs:
+        out["j_rel_l2_vs_fp64_max"] = max(errs)
+        out["worst_harmonic_dev_vs_fp64"] = max(worsts)
     print(json.dumps(out), flush=True)
 
 
