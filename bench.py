@@ -114,6 +114,8 @@ def main():
                          "stages against FP64; batch: 64 mixed instances in one batched run; serve: Poisson request "
                          "streams through the continuous-batching server")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch/serve/precision: tune every solve shape's launch configuration on its first run")
     ap.add_argument("--rates", default="0.5,2,5,20", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=16, help="serve: requests per rate")
     args = ap.parse_args()
@@ -124,6 +126,8 @@ def main():
 
     from hhgamd import lib, tbgen
 
+    if args.autotune:
+        lib.set_autotune(True)
     if args.mode == "batch":
         return bench_batch(args)
     if args.mode == "precision":
@@ -272,8 +276,9 @@ def bench_batch(args):
     seq = time.perf_counter() - t1
     kinds = {}
     for s, p in zip(solves, params):
-        kinds.setdefault(f"nb{s.nb}/k{s.kernel()}", 0)
-        kinds[f"nb{s.nb}/k{s.kernel()}"] += 1
+        L = s.launch()
+        key = f"nb{s.nb}/k{L['kernel']}" + (f"/w{L['b']}" if L["kernel"] == 6 else "")
+        kinds[key] = kinds.get(key, 0) + 1
     # plain FP64 path, one solve at a time, same instances; J(t) of each for the error check
     errs, worsts = [], []
     plain_s = 0.0
@@ -296,7 +301,7 @@ def bench_batch(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
            "data": "synthetic",
            "config": {"workload": f"batch-{len(params)}-mixed-nb248-{args.precision}", "instances": len(params),
-                      "kpoint_steps_per_pass": total, "kernels": kinds},
+                      "kpoint_steps_per_pass": total, "kernels": kinds, "autotune": bool(args.autotune)},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "one_by_one_value": total / seq, "batch_speedup_vs_one_by_one": seq / (wall / args.steps),
            "plain_fp64_one_by_one_value": total / plain_s, "vs_plain_fp64": value * plain_s / total}

@@ -67,9 +67,18 @@ double dev_last_kernel_ms(DeviceSolve* ds);
 // 0 = automatic (Hermitian-paired kernel when the blocks allow), 1 = generic.
 void dev_set_variant(DeviceSolve* ds, int variant);
 // Kernel the next run launches: 1 thread-per-k, 2 Hermitian nb=2, 3 element-per-lane.
-int dev_kernel_variant(DeviceSolve* ds);
+int dev_kernel_variant(DeviceSolve* ds, int precision);
+// Time the candidate launch configurations of this solve on its first
+// `steps` steps (state untouched) and keep the fastest; cached per shape.
+// Returns the chosen kernel kind.
+int dev_autotune(DeviceSolve* ds, int precision, int steps);
+// Process-wide switch: tune every solve's shape on its first run.
+void set_autotune(int on);
+int get_autotune();
+// Launch key the next run uses: kind, a, b, c, fp32.
+void dev_launch_key(DeviceSolve* ds, int precision, int* out5);
 // Flops per k-point and RK4 step of the formulation the next run launches.
-double dev_flops_per_kstep(DeviceSolve* ds);
+double dev_flops_per_kstep(DeviceSolve* ds, int precision);
 void dev_destroy(DeviceSolve* ds);
 
 // Batched propagation: all added solves advance nt steps in one launch per

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -1289,6 +1290,8 @@ struct DeviceSolve {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     bool use_herm = true;   // paired Hermitian kernel when the blocks allow it
     bool use_sep = true;    // row-separable kernel for Hermitian 3 <= nb <= 16
+    bool tuned[2] = {false, false};  // autotuned launch key per precision (FP64, FP32)
+    int tkey[2][4] = {};
     float last_kernel_ms = 0.f;
 };
 
@@ -1382,51 +1385,90 @@ struct LaunchKey {
     }
 };
 
-static LaunchKey classify(const DeviceSolve* ds, int precision) {
-    const SolveDesc& d = ds->d;
+// Candidate launch keys of a solve; an invalid candidate has kind 0.
+static bool reg_pairs_ok(const PairDesc& pd) {
+    return pd.vpaired && (pd.nx == 0 ? pd.nrep <= 4 : (pd.nrep <= 3 && pd.nx <= 2 && pd.nv <= 2));
+}
+static LaunchKey key_pair(const DeviceSolve* ds, int fp32) {  // coefficients in registers
     const PairDesc& pd = ds->pd;
     LaunchKey k;
-    k.fp32 = precision != 0;
-    const bool reg_pairs = pd.vpaired && (pd.nx == 0 ? pd.nrep <= 4 : (pd.nrep <= 3 && pd.nx <= 2 && pd.nv <= 2));
-    if (pd.nrep > 0 && ds->use_herm && !reg_pairs) {
-        k.kind = KIND_PAIRS;
-        k.a = pd.nrep <= 12 ? 12 : 32;
-        if (pd.nx == 0 && pd.nv <= 2) { k.b = 0; k.c = 2; }
-        else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
-        else { k.b = 3; k.c = 3; }
-        return k;
-    }
-    if (pd.nrep > 0 && reg_pairs && ds->use_herm) {
-        k.kind = KIND_PAIR;
-        k.a = pd.nrep <= 3 ? 3 : 4;
-        if (pd.nx == 0) { k.b = 0; k.c = pd.nv <= 2 ? 2 : 3; }
-        else { k.b = 2; k.c = 2; }
-        return k;
-    }
-    if (d.nb <= 2 && d.ns <= 32) {
-        k.kind = KIND_THREAD;
-        k.a = d.nb;
-        k.b = d.nb == 1 ? 16 : (d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32);
-        return k;
-    }
-    if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
-    if (d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep) {
-        // waves per workgroup: cover a k-row with equal chunks of at most 8 waves
-        const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
+    k.fp32 = fp32;
+    if (!(pd.nrep > 0 && ds->use_herm && reg_pairs_ok(pd))) return k;
+    k.kind = KIND_PAIR;
+    k.a = pd.nrep <= 3 ? 3 : 4;
+    if (pd.nx == 0) { k.b = 0; k.c = pd.nv <= 2 ? 2 : 3; }
+    else { k.b = 2; k.c = 2; }
+    return k;
+}
+static LaunchKey key_pairs(const DeviceSolve* ds, int fp32) {  // streamed coefficients
+    const PairDesc& pd = ds->pd;
+    LaunchKey k;
+    k.fp32 = fp32;
+    if (!(pd.nrep > 0 && ds->use_herm)) return k;
+    k.kind = KIND_PAIRS;
+    k.a = pd.nrep <= 12 ? 12 : 32;
+    if (pd.nx == 0 && pd.nv <= 2) { k.b = 0; k.c = 2; }
+    else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
+    else { k.b = 3; k.c = 3; }
+    return k;
+}
+static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
+    const SolveDesc& d = ds->d;
+    LaunchKey k;
+    k.fp32 = fp32;
+    if (!(d.nb <= 2 && d.ns <= 32)) return k;
+    k.kind = KIND_THREAD;
+    k.a = d.nb;
+    k.b = d.nb == 1 ? 16 : (d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32);
+    return k;
+}
+// w <= 0: waves per workgroup by the default rule (equal chunks of a k-row,
+// at most 8 waves, reduced until the LDS footprint fits)
+static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w) {
+    const SolveDesc& d = ds->d;
+    LaunchKey k;
+    k.fp32 = fp32;
+    if (!(d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep)) return k;
+    const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
+    const int maxd = d.nd <= 5 ? 5 : 8;
+    if (w <= 0) {
         const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
-        int w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
-        while (w > 1 && sep_layout(nbp, d.nd <= 5 ? 5 : 8, k.fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
-        if (sep_layout(nbp, d.nd <= 5 ? 5 : 8, k.fp32 ? 4 : 8, w).bytes <= LDS_MAX) {
-            k.kind = KIND_SEP;
-            k.a = nbp;
-            k.b = w;
-            k.c = d.nd <= 5 ? 5 : 8;
-            return k;
-        }
+        w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
+        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
     }
+    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w).bytes > LDS_MAX) return k;
+    k.kind = KIND_SEP;
+    k.a = nbp;
+    k.b = w;
+    k.c = maxd;
+    return k;
+}
+
+static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
+    const SolveDesc& d = ds->d;
+    const int fp32 = precision != 0;
+    LaunchKey k;
+    if (!reg_pairs_ok(ds->pd) && (k = key_pairs(ds, fp32)).kind) return k;
+    if ((k = key_pair(ds, fp32)).kind) return k;
+    if ((k = key_thread(ds, fp32)).kind) return k;
+    if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
+    if ((k = key_sep(ds, fp32, 0)).kind) return k;
     k.kind = KIND_ELEM;
+    k.fp32 = fp32;
     k.a = d.nb * d.nb <= 64 ? 1 : 4;
     return k;
+}
+
+// The key a run uses: the autotuned one when the solve has been tuned for
+// this precision, else the default rule.
+static LaunchKey classify(const DeviceSolve* ds, int precision) {
+    const int p = precision != 0;
+    if (ds->tuned[p]) {
+        LaunchKey k;
+        k.kind = ds->tkey[p][0]; k.a = ds->tkey[p][1]; k.b = ds->tkey[p][2]; k.c = ds->tkey[p][3]; k.fp32 = p;
+        return k;
+    }
+    return classify_default(ds, precision);
 }
 
 static int comps_of(const LaunchKey& k) {
@@ -1551,28 +1593,121 @@ static void set_lds_limit(const void* fn, size_t lds) {
 
 // Launch the propagation kernel of one solve; returns the number of current
 // components per wave partial.
-static int launch_single(DeviceSolve* ds, int precision, int& nwaves) {
-    const LaunchKey k = classify(ds, precision);
+static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, int& nwaves) {
     int wpt = 0, block = 0;
-    const int ntiles = tiles_of(ds->d, k, wpt, block);
+    const int ntiles = tiles_of(d, k, wpt, block);
     nwaves = ntiles * wpt;
     const int C = comps_of(k);
     ensure_jpart(ds, nwaves, C);
     const InstPtrs ip = ptrs_of(ds, nwaves);
-    const size_t lds = lds_of(ds->d, k);
+    const size_t lds = lds_of(d, k);
     dispatch(k, [&](auto single, auto) {
         set_lds_limit((const void*)single, lds);
         if constexpr (std::is_invocable_v<decltype(single), SolveDesc, PairDesc, InstPtrs>)
-            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, ds->d, ds->pd, ip);
+            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, d, ds->pd, ip);
         else
-            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, ds->d, ip);
+            hipLaunchKernelGGL(single, dim3(ntiles), dim3(block), lds, ds->stream, d, ip);
     });
     HIP_CHECK(hipGetLastError());
     return C;
 }
 
+static int launch_single(DeviceSolve* ds, int precision, int& nwaves) {
+    return launch_with(ds, classify(ds, precision), ds->d, nwaves);
+}
+
+// --------------------------------------------------------------------------
+// Autotuner. The candidates of a solve are the launch keys whose kernels
+// cover it: for the row-separable kernel every workgroup size of 1..8 waves
+// that fits LDS; for nb = 2 Hermitian models the register-pair, streamed-pair
+// and thread-per-k kernels. Each candidate propagates the first `steps` steps
+// from the current state into the scratch state buffer (the state and the
+// recorded current are untouched), timed with events on the solve's stream;
+// the fastest wins. Winners are cached per solve shape for the process, so a
+// batch or a server tunes each shape once.
+static std::mutex g_tune_mu;
+static std::map<std::string, LaunchKey> g_tune_cache;
+static int g_autotune = 0;
+
+static std::string shape_of(const DeviceSolve* ds, int precision) {
+    const SolveDesc& d = ds->d;
+    const PairDesc& pd = ds->pd;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", precision != 0, d.nb, d.N1,
+                  d.N2, d.ns, d.nd, d.nwd, d.herm, d.hermhx, d.xmask, d.vmask, pd.nrep, pd.nx, pd.nv, pd.vpaired,
+                  (int)ds->use_herm, (int)ds->use_sep);
+    return buf;
+}
+
+static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
+    const int fp32 = precision != 0;
+    std::vector<LaunchKey> out{classify_default(ds, precision)};
+    auto add = [&](const LaunchKey& k) {
+        if (!k.kind) return;
+        for (auto& o : out)
+            if (!(o < k) && !(k < o)) return;
+        out.push_back(k);
+    };
+    if (out[0].kind == KIND_SEP)
+        for (int w = 1; w <= 8; ++w) add(key_sep(ds, fp32, w));
+    if (ds->pd.nrep > 0 && ds->use_herm) {
+        add(key_pair(ds, fp32));
+        add(key_pairs(ds, fp32));
+        add(key_thread(ds, fp32));
+    }
+    return out;
+}
+
+static void set_tuned(DeviceSolve* ds, int p, const LaunchKey& k) {
+    ds->tuned[p] = true;
+    ds->tkey[p][0] = k.kind; ds->tkey[p][1] = k.a; ds->tkey[p][2] = k.b; ds->tkey[p][3] = k.c;
+}
+
+int dev_autotune(DeviceSolve* ds, int precision, int steps) {
+    const int p = precision != 0;
+    const std::string shape = shape_of(ds, precision);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(shape);
+        if (it != g_tune_cache.end()) {
+            set_tuned(ds, p, it->second);
+            return it->second.kind;
+        }
+    }
+    const std::vector<LaunchKey> cand = candidates(ds, precision);
+    SolveDesc d = ds->d;
+    d.nt = std::max(1, std::min(d.nt, steps));
+    LaunchKey best = cand[0];
+    float best_ms = 0.f;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the first launch loads the code object
+            int nw = 0;
+            HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
+            launch_with(ds, cand[i], d, nw);
+            HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
+            HIP_CHECK(hipEventSynchronize(ds->evk));
+            HIP_CHECK(hipEventElapsedTime(&ms, ds->ev0, ds->evk));
+        }
+        if (i == 0 || ms < best_ms) { best_ms = ms; best = cand[i]; }
+    }
+    set_tuned(ds, p, best);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[shape] = best;
+    return best.kind;
+}
+
+void set_autotune(int on) { g_autotune = on ? 1 : 0; }
+int get_autotune() { return g_autotune; }
+
+void dev_launch_key(DeviceSolve* ds, int precision, int* out5) {
+    const LaunchKey k = classify(ds, precision);
+    out5[0] = k.kind; out5[1] = k.a; out5[2] = k.b; out5[3] = k.c; out5[4] = k.fp32;
+}
+
 void dev_launch(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
+    if (g_autotune && !ds->tuned[precision != 0]) dev_autotune(ds, precision, 32);
     int nwaves = 0;
     HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
     const int C = launch_single(ds, precision, nwaves);
@@ -1653,6 +1788,8 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
     b->scratch.clear();
     std::map<LaunchKey, std::vector<int>> groups;
     std::vector<LaunchKey> keys(b->items.size());
+    for (size_t i = 0; i < b->items.size(); ++i)
+        if (g_autotune && !b->items[i]->tuned[b->prec[i] != 0]) dev_autotune(b->items[i], b->prec[i], 32);
     for (size_t i = 0; i < b->items.size(); ++i) {
         keys[i] = classify(b->items[i], b->prec[i]);
         groups[keys[i]].push_back((int)i);
@@ -1753,20 +1890,21 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
 }
 
 void dev_set_variant(DeviceSolve* ds, int variant) {
+    ds->tuned[0] = ds->tuned[1] = false;
     ds->use_herm = (variant == 0);
     ds->use_sep = (variant == 0);
 }
 
-int dev_kernel_variant(DeviceSolve* ds) { return classify(ds, 0).kind; }
+int dev_kernel_variant(DeviceSolve* ds, int precision) { return classify(ds, precision).kind; }
 
 // Arithmetic of the launched formulation per k-point and RK4 step, counting
 // a fused multiply-add as 2 flops. Hermitian nb = 2 kernel: pair phases
 // (6 nrep), H and rbc sums (16 nrep each), stage fields (24 nx), four
 // commutators (4 x 13), stage inputs (24), RK4 accumulation (15), update (8),
 // current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8).
-double dev_flops_per_kstep(DeviceSolve* ds) {
+double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
-    const int kind = dev_kernel_variant(ds);
+    const int kind = dev_kernel_variant(ds, precision);
     if (kind == KIND_PAIR || kind == KIND_PAIRS) {
         const PairDesc& pd = ds->pd;
         const double jcost = kind == KIND_PAIR ? pd.nv * (16.0 * pd.nrep + 7) : pd.nv * (32.0 * pd.nrep + 14);
@@ -1780,7 +1918,7 @@ double dev_flops_per_kstep(DeviceSolve* ds) {
         // ~26 P^2 flops, current 3 D (4 + 1) cMAC per 2x2 lane block; plus the
         // per-row tables (90 flops per support and element) amortised over
         // the k-points of a workgroup
-        const LaunchKey k = classify(ds, 0);
+        const LaunchKey k = classify(ds, precision);
         const double P = k.a, P2 = P * P, D = k.c;
         const double kpt = k.b * (64.0 / ((P / 2) * (P / 2)));
         const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);

@@ -209,8 +209,29 @@ API void SWESim_set_kernel_variant(SWESim* s, int v) {
         if (s->dev) hhg::dev_set_variant(s->dev, v);
     });
 }
-API double SWESim_flops_per_kstep(SWESim* s) { return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev) : 0.0; }
-API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev) : 0; }
+API double SWESim_flops_per_kstep(SWESim* s) {
+    return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev, s->precision) : 0.0;
+}
+API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev, s->precision) : 0; }
+API int SWESim_autotune(SWESim* s, int steps) {
+    int kind = -1;
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_autotune: call SWESim_init first");
+        if (steps < 1) throw std::runtime_error("SWESim_autotune: steps must be positive");
+        kind = hhg::dev_autotune(s->dev, s->precision, steps);
+    });
+    return s && s->status == 0 ? kind : -1;
+}
+API int SWESim_get_launch(SWESim* s, int* out5) {
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_get_launch: call SWESim_init first");
+        if (!out5) throw std::runtime_error("SWESim_get_launch: null output");
+        hhg::dev_launch_key(s->dev, s->precision, out5);
+    });
+    return s ? s->status : -1;
+}
+API void hhg_set_autotune(int on) { hhg::set_autotune(on); }
+API int hhg_get_autotune() { return hhg::get_autotune(); }
 API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_last_kernel_ms(s->dev) : 0.0; }
 
 }  // extern "C"

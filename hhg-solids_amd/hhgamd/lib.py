@@ -53,7 +53,9 @@ def bind(lib):
         "SWESim_get_fields": ([V, _D, _D, _D], I), "SWESim_get_rho_k": ([V, _C], I),
         "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
         "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
-        "SWESim_flops_per_kstep": ([V], Dd),
+        "SWESim_flops_per_kstep": ([V], Dd), "SWESim_autotune": ([V, I], I),
+        "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
+        "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
         "hhg_batch_new": ([], V), "hhg_batch_delete": ([V], None), "hhg_batch_add": ([V, V], I),
         "hhg_batch_size": ([V], I), "hhg_batch_run": ([V], I), "hhg_batch_last_ms": ([V], Dd),
         "hhg_batch_get_error": ([V], P),
@@ -63,6 +65,11 @@ def bind(lib):
         f.argtypes = args
         f.restype = res
     return lib
+
+
+def set_autotune(on, lib=None):
+    """Process-wide: tune every solve's launch configuration on its first run."""
+    bind(lib or load()).hhg_set_autotune(1 if on else 0)
 
 
 class Solve:
@@ -112,6 +119,21 @@ class Solve:
 
     def flops_per_kstep(self):
         return self.lib.SWESim_flops_per_kstep(self.ptr)
+
+    def autotune(self, steps=32):
+        """Time the launch configurations that cover this solve on its first
+        `steps` steps (state untouched) and keep the fastest; returns the
+        chosen kernel id."""
+        k = self.lib.SWESim_autotune(self.ptr, int(steps))
+        self._check("autotune")
+        return k
+
+    def launch(self):
+        """Launch configuration of the next run: dict(kernel, a, b, c, fp32)."""
+        out = np.zeros(5, dtype=np.int32)
+        self.lib.SWESim_get_launch(self.ptr, out)
+        self._check("get_launch")
+        return dict(zip(("kernel", "a", "b", "c", "fp32"), out.tolist()))
 
     def kernel_ms(self):
         return self.lib.SWESim_last_kernel_ms(self.ptr)

@@ -67,9 +67,24 @@ void SWESim_set_precision(SWESim* swe, int precision);
 /* Kernel selection: 0 = automatic (default; the Hermitian-paired nb=2 kernel
  * when every block satisfies B(-R) = B(R)^dagger), 1 = generic kernels. */
 void SWESim_set_kernel_variant(SWESim* swe, int variant);
-/* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2,
- * 3 = element per lane; 0 before SWESim_init. */
+/* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2 with
+ * register pairs, 3 = element per lane, 4 = Hermitian nb=2 with streamed
+ * pairs, 6 = row-separable (Hermitian nb >= 3); 0 before SWESim_init. */
 int SWESim_get_kernel(SWESim* swe);
+/* Autotuner: times every launch configuration that covers this solve (kernel
+ * family, workgroup size) on its first `steps` RK4 steps without changing
+ * its state or recorded current, and keeps the fastest for later runs at the
+ * current precision. Winners are cached per solve shape for the process.
+ * Returns the chosen kernel (as SWESim_get_kernel), -1 on error. */
+int SWESim_autotune(SWESim* swe, int steps);
+/* Launch configuration of the next run: out5 = {kernel, a, b, c, fp32}
+ * (template parameters; for kernel 6: padded band count, waves per
+ * workgroup, table depth). Returns 0 on success. */
+int SWESim_get_launch(SWESim* swe, int* out5);
+/* Process-wide: when on, every solve (single, batched or served) is tuned on
+ * its first run as by SWESim_autotune(swe, 32). Default off. */
+void hhg_set_autotune(int on);
+int hhg_get_autotune(void);
 /* Arithmetic of the launched formulation per k-point and RK4 step (flops,
  * a fused multiply-add counted as 2). */
 double SWESim_flops_per_kstep(SWESim* swe);
