@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size oracle J check")
-    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "batch", "serve"],
+    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "batch", "serve", "fftform"],
                     help="plain: the FP64 200x200 solve (default); precision: the 8-band 120x120 solve with FP32 "
                          "stages against FP64; batch: 64 mixed instances in one batched run; serve: Poisson request "
                          "streams through the continuous-batching server")
@@ -132,6 +132,8 @@ def main():
         return bench_batch(args)
     if args.mode == "precision":
         return bench_precision(args)
+    if args.mode == "fftform":
+        return bench_fftform(args)
     if args.mode == "serve":
         return bench_serve(args)
 
@@ -244,6 +246,50 @@ def bench_precision(args):
                         "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None, "kernel_ms": kms,
                         "kernel": KERNELS.get(kernel_id), "flops_per_kstep": F,
                         "generic_flops_per_kstep": algorithmic_flops_per_kstep(nb, 25)}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_fftform(args):
+    """The reference's real-space algorithm on the GPU (kernel variant 2:
+    rho(R), LDS-resident radix FFTs of 3 nb^2 fields per RK4 stage, one step
+    replayed as a HIP graph) on the plain workload, against the per-k path.
+    Streamed bytes per k-point and RK4 step: 98 nb^2 complex128 fields touched
+    (per stage: input forming 8, forward FFT rows + columns of 2 nb^2 fields 8,
+    commutator 3, inverse FFT 4; update 6) = 98 * 16 nb^2 bytes."""
+    from hhgamd import lib, tbgen
+    tmp = tempfile.mkdtemp(prefix="hhg_bench_")
+    path_tb = os.path.join(tmp, "haldane_tb.dat")
+    tbgen.write_tb(path_tb, *tbgen.haldane())
+    prm = workload_param(path_tb, args.nr, "fp64")
+    s = lib.Solve(dict(prm, kernel="fftform"))
+    nk, nt, nb = s.N, s.nt, s.nb
+    for _ in range(args.warmup):
+        s.run()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        s.run()
+        lat.append((time.perf_counter() - a) * 1e3)
+    wall = time.perf_counter() - t0
+    _, Jf = s.current()
+    s.close()
+    with lib.Solve(prm) as sk:
+        sk.run()
+        _, Jk = sk.current()
+    bytes_per_kstep = 98 * 16 * nb * nb
+    ms = wall / args.steps * 1e3
+    gbs = bytes_per_kstep * nk * nt / (ms * 1e-3) / 1e9
+    out = {"metric": METRIC, "value": nk * nt * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"haldane-2band-{args.nr}x{args.nr}-nt{nt}-fp64-fftform", "nk": nk, "nb": nb,
+                      "nt": nt, "kernel": "FFT form (reference algorithm), graph per RK4 step"},
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "j_rel_l2_vs_per_k": float(np.linalg.norm(Jf - Jk) / np.linalg.norm(Jk)),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+                        "bytes_per_kstep": bytes_per_kstep, "note": "whole-step streamed bytes over solve time"}}
     print(json.dumps(out), flush=True)
 
 

@@ -46,8 +46,20 @@ struct HostSolve {
     std::vector<double> pcoef;       // PAIR_HEAD + PAIR_MAX*PAIR_REP (when pd.vpaired)
     std::vector<double> scoef;       // PS_HEAD + PAIR_MAX*PS_REP
     std::vector<double> psteps;      // nt * pd.stride
+    double lat[6] = {};              // a1, a2 (Angstrom), for the supercell phase of the FFT form
 };
 HostSolve build_solve(const TBModel& m, const SolveParams& p);
+
+// The reference's real-space algorithm on the GPU (sbe_fftform.hip): rho(R),
+// 2-D FFTs in every RK4 stage, one step replayed as a HIP graph. FP64 only.
+struct FFTFormSolve;
+FFTFormSolve* fftform_create(const HostSolve& hs);
+double fftform_run(FFTFormSolve* fs, double* J_out);  // nt steps; returns device ms
+double fftform_last_ms(FFTFormSolve* fs);
+void fftform_destroy(FFTFormSolve* fs);
+// Unnormalised in-place 2-D FFTs of nch row-major n1 x n2 complex fields on
+// the GPU (host buffer in and out), the transform of the FFT form.
+void fft2_host(double* data, int nch, int n1, int n2, int sign);
 
 // Device side (sbe_kernels.hip).
 struct DeviceSolve;

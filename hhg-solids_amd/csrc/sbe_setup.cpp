@@ -339,6 +339,8 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
         std::memcpy(&hs.blk[(size_t)s * NOPS * nb2 * 2], sup[s].b.data(), sizeof(cplx) * NOPS * nb2);
     }
 
+    for (int ax = 0; ax < 3; ++ax) { hs.lat[ax] = m.lat[0][ax]; hs.lat[3 + ax] = m.lat[1][ax]; }
+
     // Supports grouped by n2 for the row-separable kernels.
     d.hermhx = herm_hx ? 1 : 0;
     {

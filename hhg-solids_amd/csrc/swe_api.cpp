@@ -42,11 +42,15 @@ struct SWESim {
     int variant = 0;
     std::unique_ptr<hhg::HostSolve> host;
     hhg::DeviceSolve* dev = nullptr;
+    hhg::FFTFormSolve* ff = nullptr;  // kernel variant 2, created on its first run
     std::vector<cplx> J[3];  // appended per run, like Observable::_data
     int status = 0;
     std::string err;
     double last_ms = 0.0;
-    ~SWESim() { hhg::dev_destroy(dev); }
+    ~SWESim() {
+        hhg::dev_destroy(dev);
+        hhg::fftform_destroy(ff);
+    }
 };
 
 static thread_local std::string g_last_error;
@@ -90,6 +94,8 @@ static void sim_init(SWESim* s, bool device = true) {
     hhg::DeviceSolve* dev = device ? hhg::dev_create(*hs) : nullptr;
     if (dev) hhg::dev_set_variant(dev, s->variant);
     hhg::dev_destroy(s->dev);
+    hhg::fftform_destroy(s->ff);
+    s->ff = nullptr;
     s->dev = dev;
     s->host = std::move(hs);
     for (auto& j : s->J) j.clear();
@@ -108,7 +114,13 @@ API void SWESim_run_simulation(SWESim* s) {
         if (!s->dev) throw std::runtime_error("SWESim_run_simulation: call SWESim_init first (SWESim_init_host sets up the host tables only)");
         const int nt = s->host->d.nt;
         std::vector<double> J((size_t)6 * nt);
-        s->last_ms = hhg::dev_run(s->dev, s->precision, J.data());
+        if (s->variant == 2) {  // the reference's real-space FFT algorithm on the GPU
+            if (s->precision != 0) throw std::runtime_error("the FFT form runs in FP64 only");
+            if (!s->ff) s->ff = hhg::fftform_create(*s->host);
+            s->last_ms = hhg::fftform_run(s->ff, J.data());
+        } else {
+            s->last_ms = hhg::dev_run(s->dev, s->precision, J.data());
+        }
         for (int a = 0; a < 3; ++a)
             for (int ti = 0; ti < nt; ++ti) s->J[a].push_back(cplx(J[(size_t)6 * ti + 2 * a], J[(size_t)6 * ti + 2 * a + 1]));
     });
@@ -204,15 +216,29 @@ API int SWESim_get_rho_k(SWESim* s, cplx* rho) {
 API double SWESim_last_run_ms(SWESim* s) { return s ? s->last_ms : 0.0; }
 API void SWESim_set_kernel_variant(SWESim* s, int v) {
     guarded(s, [&] {
-        if (v != 0 && v != 1) throw std::runtime_error("kernel variant must be 0 (automatic) or 1 (generic)");
+        if (v < 0 || v > 2)
+            throw std::runtime_error("kernel variant must be 0 (automatic), 1 (generic) or 2 (FFT form)");
         s->variant = v;
-        if (s->dev) hhg::dev_set_variant(s->dev, v);
+        if (s->dev) hhg::dev_set_variant(s->dev, v == 1 ? 1 : 0);
     });
 }
 API double SWESim_flops_per_kstep(SWESim* s) {
     return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev, s->precision) : 0.0;
 }
-API int SWESim_get_kernel(SWESim* s) { return (s && s->dev) ? hhg::dev_kernel_variant(s->dev, s->precision) : 0; }
+API int SWESim_get_kernel(SWESim* s) {
+    if (s && s->dev && s->variant == 2) return 7;
+    return (s && s->dev) ? hhg::dev_kernel_variant(s->dev, s->precision) : 0;
+}
+API int hhg_fft2(double* data, int nch, int n1, int n2, int sign) {
+    try {
+        if (!data) throw std::runtime_error("hhg_fft2: null data");
+        hhg::fft2_host(data, nch, n1, n2, sign);
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
 API int SWESim_autotune(SWESim* s, int steps) {
     int kind = -1;
     guarded(s, [&] {

@@ -56,6 +56,7 @@ def bind(lib):
         "SWESim_flops_per_kstep": ([V], Dd), "SWESim_autotune": ([V, I], I),
         "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
         "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
+        "hhg_fft2": ([np.ctypeslib.ndpointer(dtype=np.complex128, flags="C"), I, I, I, I], I),
         "hhg_batch_new": ([], V), "hhg_batch_delete": ([V], None), "hhg_batch_add": ([V, V], I),
         "hhg_batch_size": ([V], I), "hhg_batch_run": ([V], I), "hhg_batch_last_ms": ([V], Dd),
         "hhg_batch_get_error": ([V], P),
@@ -65,6 +66,18 @@ def bind(lib):
         f.argtypes = args
         f.restype = res
     return lib
+
+
+def fft2(a, sign=-1, lib=None):
+    """Unnormalised 2-D FFT over the last two axes on the GPU (the FFT form's
+    LDS-resident radix transform); returns a new complex128 array."""
+    L = bind(lib or load())
+    x = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    n1, n2 = x.shape[-2:]
+    nch = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1
+    if L.hhg_fft2(x.reshape(-1), nch, n1, n2, int(sign)) != 0:
+        raise RuntimeError("hhg_fft2: " + L.hhg_last_error().decode())
+    return x
 
 
 def set_autotune(on, lib=None):
@@ -97,7 +110,7 @@ class Solve:
         prec = {"fp64": 0, "fp32": 1}.get(param.get("precision", "fp64"), -1)  # the library rejects -1
         L.SWESim_set_precision(self.ptr, prec)
         self._check("set_precision")
-        L.SWESim_set_kernel_variant(self.ptr, {"auto": 0, "generic": 1}.get(param.get("kernel", "auto"), -1))
+        L.SWESim_set_kernel_variant(self.ptr, {"auto": 0, "generic": 1, "fftform": 2}.get(param.get("kernel", "auto"), -1))
         self._check("set_kernel_variant")
         (L.SWESim_init if device else L.SWESim_init_host)(self.ptr)
         self._check("init")

@@ -64,12 +64,17 @@ int SWESim_get_num_orb(SWESim* swe);
 /* Arithmetic of the RK4 stages: 0 = FP64 (default), 1 = FP32 stages with the
  * band basis, initial state, state update and J accumulation in FP64. */
 void SWESim_set_precision(SWESim* swe, int precision);
-/* Kernel selection: 0 = automatic (default; the Hermitian-paired nb=2 kernel
- * when every block satisfies B(-R) = B(R)^dagger), 1 = generic kernels. */
+/* Kernel selection: 0 = automatic (default; the Hermitian-specialised
+ * kernels when every H and rbc block satisfies B(-R) = B(R)^dagger),
+ * 1 = generic kernels, 2 = the reference's real-space algorithm (rho(R), 2-D
+ * FFTs of every stage in LDS-resident radix kernels, one RK4 step replayed
+ * as a HIP graph; FP64 only; it keeps its own state, started from the
+ * initial state on its first run). */
 void SWESim_set_kernel_variant(SWESim* swe, int variant);
 /* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2 with
  * register pairs, 3 = element per lane, 4 = Hermitian nb=2 with streamed
- * pairs, 6 = row-separable (Hermitian nb >= 3); 0 before SWESim_init. */
+ * pairs, 6 = row-separable (Hermitian nb >= 3), 7 = FFT form (variant 2);
+ * 0 before SWESim_init. */
 int SWESim_get_kernel(SWESim* swe);
 /* Autotuner: times every launch configuration that covers this solve (kernel
  * family, workgroup size) on its first `steps` RK4 steps without changing
@@ -85,6 +90,12 @@ int SWESim_get_launch(SWESim* swe, int* out5);
  * its first run as by SWESim_autotune(swe, 32). Default off. */
 void hhg_set_autotune(int on);
 int hhg_get_autotune(void);
+/* Unnormalised in-place 2-D FFTs on the GPU of nch row-major n1 x n2 complex
+ * fields in host memory (sign -1 forward, +1 backward): the LDS-resident
+ * mixed-radix transform of the FFT form (sides up to 256 with prime factors
+ * 2, 3, 5, 7, 11, 13). The transform of src/fftw_helper.cpp:5-33 without the
+ * 1/N of ifft3. Returns 0, or -1 with hhg_last_error set. */
+int hhg_fft2(double* data, int nch, int n1, int n2, int sign);
 /* Arithmetic of the launched formulation per k-point and RK4 step (flops,
  * a fused multiply-add counted as 2). */
 double SWESim_flops_per_kstep(SWESim* swe);

@@ -1,0 +1,70 @@
+"""The reference's real-space algorithm on the GPU (kernel variant 2): the
+LDS-resident radix FFT against numpy, the FFT-form solve against the
+oracle's FFT-form restatement and against the per-k kernels."""
+import numpy as np
+import pytest
+
+from conftest import HALDANE, HMCASE0, make_param, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+@pytest.mark.parametrize("shape", [(1, 8, 10), (3, 12, 30), (2, 200, 240), (2, 14, 26), (1, 256, 256), (4, 60, 4)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft2_matches_numpy(shape, sign):
+    from hhgamd import lib
+    rng = np.random.default_rng(sum(shape))
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    got = lib.fft2(a, sign=sign)
+    ref = np.fft.fft2(a) if sign < 0 else np.fft.ifft2(a) * (shape[1] * shape[2])
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"{shape} sign {sign}: max rel err {err:.2e}")
+    assert err <= 1e-13
+
+
+def _run(p, kernel):
+    from hhgamd import lib
+    with lib.Solve(dict(p, kernel=kernel)) as s:
+        s.run()
+        k = s.kernel()
+        return k, s.current()[1]
+
+
+@pytest.mark.parametrize("path,nr,pol", [(HMCASE0, (10, 12), (0.0, 1.0, 0.0)), (HALDANE, (12, 16), (0.6, 0.8, 0.0))])
+def test_fftform_matches_oracle_fft_form(oracle_lib, path, nr, pol):
+    p = make_param(path, nr=nr, tmax=10.0, tmax_field=8.0, pol=pol, intensity=2e12)
+    k, J = _run(p, "fftform")
+    assert k == 7
+    _, Jref = oracle_lib.run(path, p, mode="fft")
+    err = rel_l2(J, Jref)
+    print(f"FFT form vs oracle FFT form: {err:.2e}")
+    assert err <= FP64_TOL
+
+
+def test_fftform_matches_per_k_kernels(tmp_path):
+    from hhgamd import tbgen
+    path = str(tmp_path / "syn3_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(3, seed=5))
+    for pth, nr in [(HALDANE, (24, 30)), (path, (10, 14))]:
+        p = make_param(pth, nr=nr, tmax=12.0, tmax_field=10.0, pol=(0.8, 0.6, 0.0), phi=(0.3, 0.0, 0.0),
+                       intensity=3e12)
+        _, Jf = _run(p, "fftform")
+        _, Jk = _run(p, "auto")
+        err = rel_l2(Jf, Jk)
+        print(f"{nr}: FFT form vs per-k {err:.2e}")
+        assert err <= FP64_TOL
+
+
+def test_fftform_continues_and_rejects_fp32():
+    from hhgamd import lib
+    p = make_param(HMCASE0, nr=(8, 8), tmax=4.0, tmax_field=3.0)
+    with lib.Solve(dict(p, kernel="fftform")) as s:
+        s.run()
+        s.run()
+        t, J = s.current()
+        assert len(t) == s.nt and np.isfinite(J).all()
+    with pytest.raises(RuntimeError, match="FP64"):
+        with lib.Solve(dict(p, kernel="fftform", precision="fp32")) as s:
+            s.run()

@@ -145,3 +145,10 @@ def test_server_failure_paths_without_gpu(native, tmp_path):
     with pytest.raises(RuntimeError, match="precision"):
         with serve.Server() as srv:
             srv.submit(dict(make_param(HALDANE, nr=8, tmax=1.0), precision="fp8"))
+
+
+def test_fft2_rejects_unsupported_sizes_without_gpu(native):
+    with pytest.raises(RuntimeError, match="factor"):
+        native.fft2(np.zeros((17, 8), dtype=complex))
+    with pytest.raises(RuntimeError, match="at most 256"):
+        native.fft2(np.zeros((8, 512), dtype=complex))
