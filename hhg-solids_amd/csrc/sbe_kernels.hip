@@ -396,33 +396,42 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     };
     load_row(0, ph, ef);
 
-    for (int ti = 0; ti < d.nt; ++ti) {
-        double phn[NREP][2], efn[3][NXA];
-        load_row(ti + 1 < d.nt ? ti + 1 : ti, phn, efn);
-
-        cx<double> w[NREP];
+    // Phases and the field-free Fourier sums of a step depend only on the
+    // step table, so they are formed one step ahead, in the same block as the
+    // serial RK4 stage chain of the current step, whose latency they fill.
+    cx<double> w[NREP];
+    T hs[4], xs[NXA][4];
+    auto fourier = [&](const double (&a)[NREP][2], cx<double> (&wo)[NREP], T (&ho)[4], T (&xo)[NXA][4]) {
 #pragma unroll
-        for (int r = 0; r < NREP; ++r) w[r] = cmul(w0[r], cx<double>{ph[r][0], ph[r][1]});
-        // field-free part of H^(k - A): four real outputs
-        T hs[4] = {h0[0], h0[1], h0[2], h0[3]};
+        for (int r = 0; r < NREP; ++r) wo[r] = cmul(w0[r], cx<double>{a[r][0], a[r][1]});
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ho[j] = h0[j];
 #pragma unroll
         for (int r = 0; r < NREP; ++r) {
-            const T wr = (T)w[r].re, wi = (T)w[r].im;
+            const T wr = (T)wo[r].re, wi = (T)wo[r].im;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) hs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, hs[j]));
+            for (int j = 0; j < 4; ++j) ho[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, ho[j]));
         }
-        T xs[NXA][4];
 #pragma unroll
         for (int x = 0; x < NX; ++x) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xs[x][j] = x0[x][j];
+            for (int j = 0; j < 4; ++j) xo[x][j] = x0[x][j];
 #pragma unroll
             for (int r = 0; r < NREP; ++r) {
-                const T wr = (T)w[r].re, wi = (T)w[r].im;
+                const T wr = (T)wo[r].re, wi = (T)wo[r].im;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, xs[x][j]));
+                for (int j = 0; j < 4; ++j) xo[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, xo[x][j]));
             }
         }
+    };
+    fourier(ph, w, hs, xs);
+
+    for (int ti = 0; ti < d.nt; ++ti) {
+        double phn[NREP][2], efn[3][NXA];
+        load_row(ti + 1 < d.nt ? ti + 1 : ti, phn, efn);
+        cx<double> wn[NREP];
+        T hsn[4], xsn[NXA][4];
+        fourier(phn, wn, hsn, xsn);
         T A, D, Br, Bi;
         auto stage_h = [&](int j) {
             A = hs[0]; D = hs[1]; Br = hs[2]; Bi = hs[3];
@@ -471,7 +480,13 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
         JS::put(jl, lane, ti % TCH, jc);
         if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
 #pragma unroll
-        for (int r = 0; r < NREP; ++r) { ph[r][0] = phn[r][0]; ph[r][1] = phn[r][1]; }
+        for (int r = 0; r < NREP; ++r) w[r] = wn[r];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) hs[j] = hsn[j];
+#pragma unroll
+        for (int x = 0; x < NX; ++x)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xs[x][j] = xsn[x][j];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
