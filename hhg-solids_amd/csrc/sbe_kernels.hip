@@ -1434,7 +1434,7 @@ static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
     if (!(d.nb <= 2 && d.ns <= 32)) return k;
     k.kind = KIND_THREAD;
     k.a = d.nb;
-    k.b = d.nb == 1 ? 16 : (d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32);
+    k.b = d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32;  // MAXU: every support point must fit
     return k;
 }
 // w <= 0: waves per workgroup by the default rule (equal chunks of a k-row,
@@ -1560,7 +1560,7 @@ static void dispatch(const LaunchKey& k, F&& f) {
     if (k.a == NB && k.b == U)                                                                               \
         return go(k_thread_per_k<NB, U, double>, k_thread_per_k_batch<NB, U, double>, k_thread_per_k<NB, U, float>, \
                   k_thread_per_k_batch<NB, U, float>);
-            THR(1, 16) THR(2, 8) THR(2, 16) THR(2, 32)
+            THR(1, 8) THR(1, 16) THR(1, 32) THR(2, 8) THR(2, 16) THR(2, 32)
 #undef THR
             break;
         case KIND_SEP:
@@ -1609,6 +1609,7 @@ static void set_lds_limit(const void* fn, size_t lds) {
 // Launch the propagation kernel of one solve; returns the number of current
 // components per wave partial.
 static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, int& nwaves) {
+    if (k.kind == KIND_THREAD && d.ns > k.b) throw std::runtime_error("thread-per-k kernel: support exceeds MAXU");
     int wpt = 0, block = 0;
     const int ntiles = tiles_of(d, k, wpt, block);
     nwaves = ntiles * wpt;

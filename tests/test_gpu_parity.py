@@ -85,11 +85,15 @@ def test_field_free_is_stationary(native):
     assert np.abs(rho - rho0).max() < 1e-13
 
 
-@pytest.mark.parametrize("nb,nr,tmax", [(2, 12, 10.0), (3, 8, 6.0), (4, 10, 10.0), (8, 12, 10.0), (16, 6, 3.0)])
-def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax):
+# band counts: exact tiles (2, 4, 8, 16) and zero-padded ones (3 -> 4, 5 and 6 -> 8, 12 -> 16), one band;
+# nmax = 3 gives 7 distinct n2 values (the deeper per-row tables of the separable kernel)
+@pytest.mark.parametrize("nb,nr,tmax,nmax", [(1, 10, 10.0, 2), (2, 12, 10.0, 2), (3, 8, 6.0, 2), (4, 10, 10.0, 2),
+                                             (5, 8, 6.0, 2), (6, 8, 6.0, 2), (8, 12, 10.0, 2), (12, 6, 3.0, 2),
+                                             (16, 6, 3.0, 2), (4, (8, 10), 6.0, 3)])
+def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, nmax):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
-    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=nb))
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=nb, nmax=nmax))
     prm = make_param(path, nr=nr, tmax=tmax, tmax_field=tmax * 0.8, intensity=5e11, pol=(0.6, 0.8, 0.0))
     with native.Solve(prm, device=False) as s:
         rho0 = s.rho_k()
