@@ -307,11 +307,11 @@ def bench_batch(args):
     batch = lib.Batch(solves)
     for _ in range(args.warmup):
         batch.run()
-    lat = []
+    lat, dev = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         a = time.perf_counter()
-        batch.run()
+        dev.append(batch.run())
         lat.append((time.perf_counter() - a) * 1e3)
     wall = time.perf_counter() - t0
     batch.close()
@@ -349,6 +349,7 @@ def bench_batch(args):
            "config": {"workload": f"batch-{len(params)}-mixed-nb248-{args.precision}", "instances": len(params),
                       "kpoint_steps_per_pass": total, "kernels": kinds, "autotune": bool(args.autotune)},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "device_ms_per_pass": float(np.mean(dev)),
            "one_by_one_value": total / seq, "batch_speedup_vs_one_by_one": seq / (wall / args.steps),
            "plain_fp64_one_by_one_value": total / plain_s, "vs_plain_fp64": value * plain_s / total}
     if errs:

@@ -1768,7 +1768,10 @@ struct Batch {
     std::vector<int> prec;
     std::vector<hipStream_t> streams;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    std::vector<void*> scratch;  // device tables of the last run
+    // device tables per launch group (instances, tiles, reduction tiles),
+    // kept across runs and grown when a run needs more
+    std::vector<void*> scratch;
+    std::vector<size_t> scratch_bytes;
     float last_ms = 0.f;
     ~Batch() {
         for (void* p : scratch) (void)hipFree(p);
@@ -1800,8 +1803,6 @@ int batch_size(Batch* b) { return (int)b->items.size(); }
 
 double batch_run(Batch* b, const std::vector<double*>& J_out) {
     if (b->items.empty()) return 0.0;
-    for (void* p : b->scratch) HIP_CHECK(hipFree(p));
-    b->scratch.clear();
     std::map<LaunchKey, std::vector<int>> groups;
     std::vector<LaunchKey> keys(b->items.size());
     for (size_t i = 0; i < b->items.size(); ++i)
@@ -1817,15 +1818,25 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
     }
     // finish any pending single-solve work before the batch reads the states
     for (auto* ds : b->items) HIP_CHECK(hipStreamSynchronize(ds->stream));
-    HIP_CHECK(hipDeviceSynchronize());
+    // table buffers: slot 3 g + {0, 1, 2} of group g; a buffer that is too
+    // small is replaced (its previous run has completed: the last run ended
+    // with a synchronisation of stream 0, which waited for every group)
+    auto devalloc = [&](size_t slot, size_t bytes) {
+        bytes = std::max<size_t>(bytes, 16);
+        if (b->scratch.size() <= slot) {
+            b->scratch.resize(slot + 1, nullptr);
+            b->scratch_bytes.resize(slot + 1, 0);
+        }
+        if (b->scratch_bytes[slot] < bytes) {
+            if (b->scratch[slot]) HIP_CHECK(hipFree(b->scratch[slot]));
+            b->scratch[slot] = nullptr;
+            HIP_CHECK(hipMalloc(&b->scratch[slot], bytes));
+            b->scratch_bytes[slot] = bytes;
+        }
+        return b->scratch[slot];
+    };
     HIP_CHECK(hipEventRecord(b->e0, b->streams[0]));
     for (size_t g = 1; g < groups.size(); ++g) HIP_CHECK(hipStreamWaitEvent(b->streams[g], b->e0, 0));
-    auto devalloc = [&](size_t bytes) {
-        void* p = nullptr;
-        HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
-        b->scratch.push_back(p);
-        return p;
-    };
     size_t gi = 0;
     for (auto& [key, idx] : groups) {
         hipStream_t st = b->streams[gi++];
@@ -1872,9 +1883,10 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
             }
             pos = end;
         }
-        DevInst* d_insts = (DevInst*)devalloc(sizeof(DevInst) * insts.size());
-        int2* d_tiles = (int2*)devalloc(sizeof(int2) * tiles.size());
-        int2* d_rt = (int2*)devalloc(sizeof(int2) * rt.size());
+        const size_t g3 = 3 * (gi - 1);
+        DevInst* d_insts = (DevInst*)devalloc(g3, sizeof(DevInst) * insts.size());
+        int2* d_tiles = (int2*)devalloc(g3 + 1, sizeof(int2) * tiles.size());
+        int2* d_rt = (int2*)devalloc(g3 + 2, sizeof(int2) * rt.size());
         HIP_CHECK(hipMemcpyAsync(d_insts, insts.data(), sizeof(DevInst) * insts.size(), hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(d_rt, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice, st));

@@ -300,6 +300,7 @@ API void hhg_batch_delete(hhg_batch* b) { delete b; }
 API int hhg_batch_add(hhg_batch* b, SWESim* s) {
     return bguard(b, [&] {
         if (!s || !s->dev) throw std::runtime_error("hhg_batch_add: solve not initialised on the device");
+        if (s->variant == 2) throw std::runtime_error("hhg_batch_add: the FFT form (kernel variant 2) runs alone");
         hhg::batch_add(b->b, s->dev, s->precision);
         b->sims.push_back(s);
     });

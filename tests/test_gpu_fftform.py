@@ -68,3 +68,11 @@ def test_fftform_continues_and_rejects_fp32():
     with pytest.raises(RuntimeError, match="FP64"):
         with lib.Solve(dict(p, kernel="fftform", precision="fp32")) as s:
             s.run()
+
+
+def test_fftform_solve_cannot_join_a_batch():
+    from hhgamd import lib
+    p = make_param(HMCASE0, nr=(8, 8), tmax=2.0, tmax_field=1.5)
+    with lib.Solve(dict(p, kernel="fftform")) as s:
+        with pytest.raises(RuntimeError, match="runs alone"):
+            lib.Batch([s])
