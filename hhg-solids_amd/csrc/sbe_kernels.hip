@@ -869,6 +869,8 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     return L;
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 // Orders the LDS accesses of one wave: writes before the barrier are seen by
 // every lane's reads after it (LDS executes a wave's instructions in order).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -1084,21 +1086,47 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             TT::st2(bufX, i0 + 1, j0, rs[2], rs[3]);
             wave_lds_sync();
             cx<T> M[4];
+            if constexpr (NBP == 16 && std::is_same<T, float>::value) {
+                // 16 x 16 complex product on the matrix cores, one k-point per
+                // wave: Re M = Hr Xr - Hi Xi, Im M = Hr Xi + Hi Xr as four real
+                // 16x16x16 products of four v_mfma_f32_16x16x4f32 each. Lane l
+                // supplies A[l%16][kc4 + l/16] = H and B[kc4 + l/16][l%16] = X and
+                // receives D[4 (l/16) + r][l%16], r = 0..3.
+                const int ai = lane & 15, kq = lane >> 4;
+                f32x4 accR = {0.f, 0.f, 0.f, 0.f}, accI = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) M[q] = {(T)0, (T)0};
+                for (int kc = 0; kc < 4; ++kc) {
+                    const int kk = 4 * kc + kq;
+                    const cx<float> hc = bufH[TT::off(kk, ai)];  // H[ai][kk] = conj(H[kk][ai])
+                    const cx<float> x = bufX[TT::off(kk, ai)];   // X[kk][ai]
+                    accR = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.re, x.re, accR, 0, 0, 0);
+                    accR = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.im, x.im, accR, 0, 0, 0);   // -Hi Xi, Hi = -hc.im
+                    accI = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.re, x.im, accI, 0, 0, 0);
+                    accI = __builtin_amdgcn_mfma_f32_16x16x4f32(-hc.im, x.re, accI, 0, 0, 0);  // Hi Xr
+                }
+                wave_lds_sync();
 #pragma unroll
-            for (int m = 0; m < NBP; ++m) {
-                cx<T> h0, h1, r0, r1;
-                TT::ld2(bufH, m, i0, h0, h1);  // H[m][i0..i0+1] = conj(H[i0..i0+1][m])
-                TT::ld2(bufX, m, j0, r0, r1);  // X[m][j0..j0+1]
-                cfmac(M[0], h0, r0);
-                cfmac(M[1], h0, r1);
-                cfmac(M[2], h1, r0);
-                cfmac(M[3], h1, r1);
+                for (int r = 0; r < 4; ++r) bufH[TT::off(4 * kq + r, ai)] = cx<float>{accR[r], accI[r]};
+                wave_lds_sync();
+                TT::ld2(bufH, i0, j0, M[0], M[1]);
+                TT::ld2(bufH, i0 + 1, j0, M[2], M[3]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) M[q] = {(T)0, (T)0};
+#pragma unroll
+                for (int m = 0; m < NBP; ++m) {
+                    cx<T> h0, h1, r0, r1;
+                    TT::ld2(bufH, m, i0, h0, h1);  // H[m][i0..i0+1] = conj(H[i0..i0+1][m])
+                    TT::ld2(bufX, m, j0, r0, r1);  // X[m][j0..j0+1]
+                    cfmac(M[0], h0, r0);
+                    cfmac(M[1], h0, r1);
+                    cfmac(M[2], h1, r0);
+                    cfmac(M[3], h1, r1);
+                }
+                wave_lds_sync();
+                TT::st2(bufH, i0, j0, M[0], M[1]);
+                TT::st2(bufH, i0 + 1, j0, M[2], M[3]);
             }
-            wave_lds_sync();
-            TT::st2(bufH, i0, j0, M[0], M[1]);
-            TT::st2(bufH, i0 + 1, j0, M[2], M[3]);
             wave_lds_sync();
             cx<T> t0, t1, u0, u1;
             TT::ld2(bufH, j0, i0, t0, t1);      // M[j0][i0], M[j0][i0+1]
