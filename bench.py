@@ -121,7 +121,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
-              "(the k-grid would shard by k-slabs with one final reduce of J, see DESIGN.md)", file=sys.stderr)
+              "(the k-grid would shard by slabs of k-rows, with one final reduce of J over the GPUs)", file=sys.stderr)
         sys.exit(2)
 
     from hhgamd import lib, tbgen
