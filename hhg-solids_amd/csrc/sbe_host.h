@@ -23,6 +23,9 @@ struct TBModel {
     std::vector<cplx> r[3];          // nR*nb*nb, Angstrom
 };
 TBModel read_tb(const std::string& path);
+// Hermitian eigenproblem of the n x n row-major matrix a by cyclic Jacobi
+// rotations: a ends (numerically) diagonal, q holds the eigenvectors as columns.
+void herm_jacobi(int n, std::vector<cplx>& a, std::vector<cplx>& q);
 
 // Parameters of one solve; same meaning and units as include/settings.h:5-31.
 struct SolveParams {
@@ -60,6 +63,11 @@ void fftform_destroy(FFTFormSolve* fs);
 // Unnormalised in-place 2-D FFTs of nch row-major n1 x n2 complex fields on
 // the GPU (host buffer in and out), the transform of the FFT form.
 void fft2_host(double* data, int nch, int n1, int n2, int sign);
+
+// The text dumps of SWESim::test_files (src/swe.cpp:131-181) into test_dir
+// and fields_dir; the r <-> k transforms run on the GPU (fft2_host).
+void write_test_files(const TBModel& m, const HostSolve& hs, const std::string& test_dir,
+                      const std::string& fields_dir);
 
 // Device side (sbe_kernels.hip).
 struct DeviceSolve;

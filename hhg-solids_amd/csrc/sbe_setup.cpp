@@ -92,6 +92,8 @@ int wrap_centered(int n, int N) {  // map n to [-N/2, N/2-1] modulo N
     return i - N / 2;
 }
 
+}  // namespace
+
 // Hermitian eigenproblem by cyclic Jacobi rotations; Q gets the eigenvectors
 // as columns. Only the occupied-subspace projector is used downstream, which
 // does not depend on the eigenvector phases or on degenerate mixing.
@@ -134,6 +136,8 @@ void herm_jacobi(int n, std::vector<cplx>& a, std::vector<cplx>& q) {
             }
     }
 }
+
+namespace {
 
 // rho = Q diag(occ) Q^dagger with occ_n = [Re(Q^dagger H Q)_nn < 0]
 void t0_projector(int nb, const cplx* hk, cplx* rho) {

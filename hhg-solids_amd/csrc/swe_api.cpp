@@ -9,6 +9,7 @@
 // Unlike the reference, no exception or exit crosses the ABI: errors are
 // recorded on the handle and in a thread-local last-error string.
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -126,9 +127,14 @@ API void SWESim_run_simulation(SWESim* s) {
     });
 }
 
-// Debug text dumps of src/swe.cpp:131-181 are not part of this library.
+// Text dumps of src/swe.cpp:131-181 into test/ and fields/ under the working
+// directory (both created when missing); the solve's state is not changed.
 API void SWESim_test_files(SWESim* s) {
-    guarded(s, [&] { throw std::runtime_error("SWESim_test_files: debug dumps are not provided by this build"); });
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_test_files: call SWESim_init first");
+        std::printf("Writing test files\n");
+        hhg::write_test_files(hhg::read_tb(s->path_tb), *s->host, "test", "fields");
+    });
 }
 
 API void SWESim_set_path_tb(SWESim* s, char* path) {

@@ -152,3 +152,25 @@ def test_fft2_rejects_unsupported_sizes_without_gpu(native):
         native.fft2(np.zeros((17, 8), dtype=complex))
     with pytest.raises(RuntimeError, match="at most 256"):
         native.fft2(np.zeros((8, 512), dtype=complex))
+
+
+def test_library_has_no_undefined_project_symbols(native):
+    out = subprocess.run(["nm", "-D", "-C", "--undefined-only", native.LIB_PATH], capture_output=True, text=True,
+                         check=True)
+    assert not [l for l in out.stdout.splitlines() if "hhg::" in l]
+
+
+def test_standalone_driver_reports_errors(native, tmp_path):
+    exe = os.path.join(PKG, "hhg_swe")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe, str(tmp_path / "missing_tb.dat"), "nr1=8", "nr2=8"], capture_output=True, text=True)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = subprocess.run([exe, HMCASE0, "bogus=1"], capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown argument" in r.stderr
+
+
+def test_test_files_needs_device_init(native):
+    with native.Solve(make_param(HMCASE0, nr=8, tmax=1.0), device=False) as s:
+        s.lib.SWESim_test_files(s.ptr)
+        assert s.lib.SWESim_get_status(s.ptr) != 0
+        assert b"SWESim_init" in s.lib.SWESim_get_error(s.ptr)
