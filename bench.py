@@ -246,6 +246,11 @@ def bench_precision(args):
                         "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None, "kernel_ms": kms,
                         "kernel": KERNELS.get(kernel_id), "flops_per_kstep": F,
                         "generic_flops_per_kstep": algorithmic_flops_per_kstep(nb, 25)}}
+    prof = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        tr = json.load(open(prof)).get(out["config"]["workload"])
+        if tr:
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
     print(json.dumps(out), flush=True)
 
 
