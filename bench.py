@@ -118,6 +118,8 @@ def main():
                     help="batch/serve/precision: tune every solve shape's launch configuration on its first run")
     ap.add_argument("--rates", default="0.5,2,5,20", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=16, help="serve: requests per rate")
+    ap.add_argument("--policy", default="least-work", choices=["least-work", "fifo"],
+                    help="serve: admission order among prepared requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
@@ -371,7 +373,7 @@ def bench_serve(args):
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
-    with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8) as srv:
+    with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8, policy=args.policy) as srv:
         warm = workloads.c5(tmp, 2, rate=100.0, seed=99, precision=args.precision)
         serve.run_stream(srv, [(0.0, p) for _, p in warm])
         for i, rate in enumerate(rates):
@@ -380,13 +382,25 @@ def bench_serve(args):
             summ["rate_per_s"] = rate
             sweep.append(summ)
             print(json.dumps({"serve_rate": rate, **summ}), file=sys.stderr, flush=True)
+    # the requests of the highest-rate stream one by one on the plain FP64 path
+    from hhgamd import lib
+    top = len(rates) - 1
+    plain_s, plain_ksteps = 0.0, 0
+    for _, p in workloads.c5(tmp, args.requests, rate=rates[top], seed=50 + top, precision="fp64"):
+        with lib.Solve(p) as s:
+            a = time.perf_counter()
+            s.run()
+            plain_s += time.perf_counter() - a
+            plain_ksteps += s.N * s.nt
     best = max(sweep, key=lambda x: x["ksteps_per_s"])
     out = {"metric": METRIC, "value": best["ksteps_per_s"], "unit": UNIT, "n_gpus": 1, "steps": args.requests,
            "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
            "config": {"workload": f"serve-poisson-nb2-16-{args.precision}", "requests_per_rate": args.requests,
-                      "rates_per_s": rates},
+                      "rates_per_s": rates, "policy": args.policy},
            "latency_ms": {"p50": best["p50_latency_s"] * 1e3, "p99": best["p99_latency_s"] * 1e3},
+           "plain_fp64_one_by_one_value": plain_ksteps / plain_s,
+           "vs_plain_fp64_at_top_rate": sweep[top]["ksteps_per_s"] / (plain_ksteps / plain_s),
            "sweep": sweep}
     print(json.dumps(out), flush=True)
 

@@ -52,6 +52,13 @@ struct ServerRequest {
     long long ksteps = 0;
 };
 
+// Device work of a solve in flops of the generic per-k formulation
+// 8 (8 nb^3 + 4 ns nb^2 + 48 nb^2) per k-point and RK4 step.
+static double work_estimate(const HostSolve& hs) {
+    const double nb = hs.d.nb;
+    return (double)hs.d.N * hs.d.nt * 8.0 * (8 * nb * nb * nb + 4 * hs.d.ns * nb * nb + 48 * nb * nb);
+}
+
 struct Server::Impl {
     ServerConfig cfg;
     std::mutex mu;
@@ -119,9 +126,23 @@ struct Server::Impl {
                 if (stop && prepared.empty() && inflight.empty() && to_prepare.empty()) return;
                 size_t used = bytes_in_use;
                 int slots = cfg.max_inflight - (int)inflight.size();
-                while (!prepared.empty() && slots > 0 && used + prepared.front()->bytes <= cfg.hbm_budget) {
-                    auto r = prepared.front();
-                    prepared.pop_front();
+                if (cfg.policy == 1 && prepared.size() > 1) {
+                    // least estimated work first, with aging
+                    const double t = now_s();
+                    auto score = [&](const std::shared_ptr<ServerRequest>& r) {
+                        return work_estimate(*r->host) / (1.0 + (t - r->t_prepared) / cfg.aging_s);
+                    };
+                    std::stable_sort(prepared.begin(), prepared.end(),
+                                     [&](const auto& a, const auto& b) { return score(a) < score(b); });
+                }
+                for (auto it = prepared.begin(); it != prepared.end() && slots > 0;) {
+                    if (used + (*it)->bytes > cfg.hbm_budget) {
+                        if (cfg.policy == 0) break;  // FIFO: wait for memory in order
+                        ++it;
+                        continue;
+                    }
+                    auto r = *it;
+                    it = prepared.erase(it);
                     used += r->bytes;
                     --slots;
                     r->state = REQ_RUNNING;
@@ -240,6 +261,14 @@ long long Server::submit(const std::string& path_tb, const SolveParams& p, int p
     impl_->stats.submitted++;
     impl_->cv_prep.notify_one();
     return r->id;
+}
+
+void Server::set_policy(int policy, double aging_s) {
+    if (policy != 0 && policy != 1) throw std::runtime_error("admission policy must be 0 (arrival order) or 1 (least work first)");
+    if (!(aging_s > 0)) throw std::runtime_error("aging time must be positive");
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    impl_->cfg.policy = policy;
+    impl_->cfg.aging_s = aging_s;
 }
 
 static std::shared_ptr<ServerRequest> find(Server::Impl* im, long long id) {

@@ -14,6 +14,11 @@ struct ServerConfig {
     size_t hbm_budget = (size_t)64 << 30;  // bytes of device memory the server may hold
     int max_inflight = 8;                  // solves on the device at once (one HIP stream each)
     int host_threads = 4;                  // host preparation workers
+    // admission order among prepared requests: 0 = arrival (FIFO), 1 = least
+    // estimated device work first, the estimate divided by 1 + wait / aging_s
+    // so that a large request is not passed over indefinitely
+    int policy = 1;
+    double aging_s = 2.0;
 };
 
 struct ServerStats {
@@ -35,6 +40,8 @@ class Server {
     void timing(long long id, double* out7);
     void release(long long id);
     ServerStats stats();
+    // admission order (ServerConfig::policy, aging_s); applies from the next admission
+    void set_policy(int policy, double aging_s);
     struct Impl;
 
   private:

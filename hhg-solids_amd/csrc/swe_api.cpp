@@ -366,6 +366,12 @@ API hhg_server* hhg_server_new(double hbm_budget_gb, int max_inflight, int host_
     }
 }
 API void hhg_server_delete(hhg_server* s) { delete s; }
+API int hhg_server_set_policy(hhg_server* s, int policy, double aging_s) {
+    return sguard([&] {
+        if (!s) throw std::runtime_error("null server");
+        s->srv->set_policy(policy, aging_s);
+    });
+}
 API long long hhg_server_submit(hhg_server* s, const char* path_tb, Settings* st, int precision) {
     long long id = -1;
     sguard([&] {

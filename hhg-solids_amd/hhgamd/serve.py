@@ -28,7 +28,7 @@ def _bind(L):
         "hhg_server_state": ([V, LL], I), "hhg_server_nt": ([V, LL], I),
         "hhg_server_result": ([V, LL, _D, _C, _C, _C], I), "hhg_server_timing": ([V, LL, _D], I),
         "hhg_server_release": ([V, LL], I), "hhg_server_stats": ([V, _D], I),
-        "hhg_server_error": ([V, LL], P),
+        "hhg_server_error": ([V, LL], P), "hhg_server_set_policy": ([V, I, D], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -38,12 +38,17 @@ def _bind(L):
 
 
 class Server:
-    def __init__(self, hbm_budget_gb=64.0, max_inflight=8, host_threads=4):
+    def __init__(self, hbm_budget_gb=64.0, max_inflight=8, host_threads=4, policy="least-work", aging_s=2.0):
         self.lib = _bind(_lib.bind(_lib.load()))
         self.ptr = self.lib.hhg_server_new(float(hbm_budget_gb), int(max_inflight), int(host_threads))
         if not self.ptr:
             raise RuntimeError("hhg_server_new: " + self.lib.hhg_last_error().decode())
         self._lock = threading.Lock()
+        code = {"fifo": 0, "least-work": 1}.get(policy, -1)
+        if self.lib.hhg_server_set_policy(self.ptr, code, float(aging_s)) != 0:
+            err = self.lib.hhg_last_error().decode()
+            self.close()
+            raise RuntimeError("hhg_server_set_policy: " + err)
 
     def submit(self, param):
         L = self.lib

@@ -134,6 +134,11 @@ const char* hhg_batch_get_error(hhg_batch* b);
 typedef struct hhg_server hhg_server;
 hhg_server* hhg_server_new(double hbm_budget_gb, int max_inflight, int host_threads);
 void hhg_server_delete(hhg_server* s);  /* completes all submitted work first */
+/* Admission order among prepared requests: 0 = arrival order, 1 = least
+ * estimated device work first (default), the estimate divided by
+ * 1 + wait / aging_s so that large requests are still admitted. Returns 0,
+ * or -1 with hhg_last_error set. */
+int hhg_server_set_policy(hhg_server* s, int policy, double aging_s);
 long long hhg_server_submit(hhg_server* s, const char* path_tb, Settings* settings, int precision); /* -1: error */
 int hhg_server_wait(hhg_server* s, long long id, double timeout_s);  /* state; timeout < 0 waits */
 int hhg_server_state(hhg_server* s, long long id);

@@ -69,3 +69,17 @@ def test_server_failures_are_per_request(tmp_path):
         assert srv.wait(ok) == "done"
         t, J = srv.result(ok)
         assert len(t) == int(2.0 / 0.02197) and np.isfinite(J).all()
+
+
+def test_admission_policies_give_the_same_results(tmp_path):
+    """Arrival order and least-work-first admission change only when a request
+    runs, never its result."""
+    from hhgamd import serve
+    params = _requests(tmp_path)[:6]
+    out = {}
+    for policy in ("fifo", "least-work"):
+        with serve.Server(hbm_budget_gb=8.0, max_inflight=3, host_threads=4, policy=policy) as srv:
+            ids = [srv.submit(p) for p in params]
+            out[policy] = [srv.result(i)[1] for i in ids]
+    for a, b in zip(out["fifo"], out["least-work"]):
+        assert np.array_equal(a, b)

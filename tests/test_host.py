@@ -174,3 +174,11 @@ def test_test_files_needs_device_init(native):
         s.lib.SWESim_test_files(s.ptr)
         assert s.lib.SWESim_get_status(s.ptr) != 0
         assert b"SWESim_init" in s.lib.SWESim_get_error(s.ptr)
+
+
+def test_server_rejects_bad_policy(native):
+    from hhgamd import serve
+    with pytest.raises(RuntimeError, match="admission policy"):
+        serve.Server(policy="random")
+    with pytest.raises(RuntimeError, match="aging"):
+        serve.Server(aging_s=0.0)
