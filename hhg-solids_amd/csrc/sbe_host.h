@@ -51,7 +51,9 @@ struct HostSolve {
     std::vector<double> psteps;      // nt * pd.stride
     double lat[6] = {};              // a1, a2 (Angstrom), for the supercell phase of the FFT form
 };
-HostSolve build_solve(const TBModel& m, const SolveParams& p);
+// host_state = false leaves rho0 empty: dev_create then computes the initial
+// state on the device (same H_k, Jacobi eigensolver and T = 0 occupation).
+HostSolve build_solve(const TBModel& m, const SolveParams& p, bool host_state = true);
 
 // The reference's real-space algorithm on the GPU (sbe_fftform.hip): rho(R),
 // 2-D FFTs in every RK4 stage, one step replayed as a HIP graph. FP64 only.

@@ -1319,6 +1319,101 @@ __global__ __launch_bounds__(64) void k_reduce_j_batch(const DevInst* __restrict
 }
 
 // --------------------------------------------------------------------------
+// Initial state on the device, one thread per k (FP64): H_k from the support
+// blocks, the cyclic complex Jacobi eigensolver of the host setup, occupation
+// [(Q^dagger H_k Q)_nn < 0] and rho = Q diag(occ) Q^dagger (the T = 0 state of
+// src/rdm.cpp:23-38 in the W gauge). H, A, Q are element-major scratch
+// [nb*nb][N] so that consecutive k-points coalesce.
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+__global__ __launch_bounds__(128) void k_init_state(const SolveDesc d, const double* __restrict__ blk,
+                                                    const double* __restrict__ tw1, const double* __restrict__ tw2,
+                                                    double2* __restrict__ H, double2* __restrict__ A,
+                                                    double2* __restrict__ Q, double* __restrict__ rho_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.N) return;
+    const int nb = d.nb, nb2 = nb * nb, N = d.N;
+    const int m1 = k / d.N2, m2 = k - m1 * d.N2;
+    auto at = [&](double2* X, int i, int j) -> double2& { return X[(size_t)(i * nb + j) * N + k]; };
+    for (int e = 0; e < nb2; ++e) H[(size_t)e * N + k] = make_double2(0.0, 0.0);
+    for (int s = 0; s < d.ns; ++s) {
+        if (!(d.opmask[s] & (1u << OP_H))) continue;
+        const int j1 = posmod((long)m1 * d.n1[s], d.N1), j2 = posmod((long)m2 * d.n2[s], d.N2);
+        const double2 w = zmul(make_double2(tw1[2 * j1], tw1[2 * j1 + 1]), make_double2(tw2[2 * j2], tw2[2 * j2 + 1]));
+        const double* b = blk + 2 * ((size_t)s * NOPS + OP_H) * nb2;
+        for (int e = 0; e < nb2; ++e)
+            H[(size_t)e * N + k] = zadd(H[(size_t)e * N + k], zmul(make_double2(b[2 * e], b[2 * e + 1]), w));
+    }
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) {
+            at(A, i, j) = at(H, i, j);
+            at(Q, i, j) = make_double2(i == j ? 1.0 : 0.0, 0.0);
+        }
+    for (int it = 0; it < 60; ++it) {
+        double off = 0, on = 0;
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) {
+                const double2 x = at(A, i, j);
+                (i == j ? on : off) += x.x * x.x + x.y * x.y;
+            }
+        if (off <= 1e-32 * on || off == 0.0) break;
+        for (int p = 0; p < nb - 1; ++p)
+            for (int r = p + 1; r < nb; ++r) {
+                const double2 apr = at(A, p, r);
+                const double g = hypot(apr.x, apr.y);
+                if (g == 0.0) continue;
+                const double2 u = make_double2(apr.x / g, apr.y / g);
+                const double app = at(A, p, p).x, arr = at(A, r, r).x;
+                const double th = 0.5 * (arr - app) / g;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                // unitary G acting on columns p, r: [c, s; -s conj(u), c conj(u)]
+                const double2 g00 = make_double2(c, 0.0), g01 = make_double2(sn, 0.0);
+                const double2 g10 = make_double2(-sn * u.x, sn * u.y), g11 = make_double2(c * u.x, -c * u.y);
+                for (int i = 0; i < nb; ++i) {
+                    double2 x = at(A, i, p), y = at(A, i, r);
+                    at(A, i, p) = zadd(zmul(x, g00), zmul(y, g10));
+                    at(A, i, r) = zadd(zmul(x, g01), zmul(y, g11));
+                    x = at(Q, i, p); y = at(Q, i, r);
+                    at(Q, i, p) = zadd(zmul(x, g00), zmul(y, g10));
+                    at(Q, i, r) = zadd(zmul(x, g01), zmul(y, g11));
+                }
+                for (int j = 0; j < nb; ++j) {
+                    const double2 x = at(A, p, j), y = at(A, r, j);
+                    at(A, p, j) = zadd(zmul(zconj(g00), x), zmul(zconj(g10), y));
+                    at(A, r, j) = zadd(zmul(zconj(g01), x), zmul(zconj(g11), y));
+                }
+                at(A, p, r) = make_double2(0.0, 0.0);
+                at(A, r, p) = make_double2(0.0, 0.0);
+                at(A, p, p) = make_double2(at(A, p, p).x, 0.0);
+                at(A, r, r) = make_double2(at(A, r, r).x, 0.0);
+            }
+    }
+    // occupation from the diagonal of Q^dagger H Q with the original H; the
+    // occupied flags go into the (now unused) diagonal of A
+    for (int n = 0; n < nb; ++n) {
+        double dn = 0.0;
+        for (int i = 0; i < nb; ++i) {
+            double2 hq = make_double2(0.0, 0.0);
+            for (int j = 0; j < nb; ++j) hq = zadd(hq, zmul(at(H, i, j), at(Q, j, n)));
+            dn += zmul(zconj(at(Q, i, n)), hq).x;
+        }
+        at(A, n, n) = make_double2(dn < 0.0 ? 1.0 : 0.0, 0.0);
+    }
+    for (int i = 0; i < nb; ++i)
+        for (int j = 0; j < nb; ++j) {
+            double2 sum = make_double2(0.0, 0.0);
+            for (int n = 0; n < nb; ++n)
+                if (at(A, n, n).x != 0.0) sum = zadd(sum, zmul(at(Q, i, n), zconj(at(Q, j, n))));
+            rho_out[2 * ((size_t)(i * nb + j) * N + k)] = sum.x;
+            rho_out[2 * ((size_t)(i * nb + j) * N + k) + 1] = sum.y;
+        }
+}
+
 struct DeviceSolve {
     SolveDesc d;
     double dt = 0;
@@ -1368,14 +1463,25 @@ DeviceSolve* dev_create(const HostSolve& hs) {
         ds->steptab = upload(hs.steptab, ds->stream);
         ds->tw1 = upload(hs.tw1, ds->stream);
         ds->tw2 = upload(hs.tw2, ds->stream);
-        ds->rho = upload(hs.rho0, ds->stream);
+        if (!hs.rho0.empty()) {
+            ds->rho = upload(hs.rho0, ds->stream);
+        } else {  // initial state on the device
+            const SolveDesc& d = hs.d;
+            const size_t plane = (size_t)d.nb * d.nb * d.N;
+            ds->rho = dalloc(sizeof(double) * 2 * plane, ds->stream);
+            double2* scratch = (double2*)dalloc(sizeof(double2) * 3 * plane, ds->stream);
+            hipLaunchKernelGGL(k_init_state, dim3((d.N + 127) / 128), dim3(128), 0, ds->stream, d, ds->blk, ds->tw1,
+                               ds->tw2, scratch, scratch + plane, scratch + 2 * plane, ds->rho);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipFreeAsync(scratch, ds->stream));
+        }
         ds->pd = hs.pd;
         if (hs.pd.nrep > 0) {
             if (!hs.pcoef.empty()) ds->pcoef = upload(hs.pcoef, ds->stream);
             ds->scoef = upload(hs.scoef, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
         }
-        ds->rho_tmp = dalloc(sizeof(double) * hs.rho0.size(), ds->stream);
+        ds->rho_tmp = dalloc(sizeof(double) * 2 * (size_t)hs.d.nb * hs.d.nb * hs.d.N, ds->stream);
         ds->J = dalloc(sizeof(double) * 6 * (size_t)hs.d.nt, ds->stream);
         HIP_CHECK(hipHostMalloc((void**)&ds->hJ, sizeof(double) * 6 * (size_t)hs.d.nt, hipHostMallocDefault));
         HIP_CHECK(hipStreamSynchronize(ds->stream));
@@ -1407,7 +1513,11 @@ size_t dev_bytes(const HostSolve& hs) {
     int G = 1;
     while (G < E && G < 64) G <<= 1;
     const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
-    return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * hs.rho0.size() +
+    // rho and its scratch twin (2 x 2 nb^2 N doubles); a device-side initial
+    // state holds 3 more nb^2 N complex planes while it runs
+    const size_t rho = 2 * (size_t)E * d.N;
+    const size_t init = hs.rho0.empty() ? 6 * (size_t)E * d.N : 0;
+    return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * rho + init +
                              hs.pcoef.size() + hs.scoef.size() + hs.psteps.size() + 6 * (size_t)d.nt +
                              6 * (size_t)d.nt * waves);
 }

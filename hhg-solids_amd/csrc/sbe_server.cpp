@@ -89,7 +89,7 @@ struct Server::Impl {
             size_t bytes = 0;
             try {
                 TBModel m = read_tb(r->path_tb);
-                hs = std::make_unique<HostSolve>(build_solve(m, r->params));
+                hs = std::make_unique<HostSolve>(build_solve(m, r->params, false));  // initial state on the device
                 bytes = dev_bytes(*hs);
                 if (bytes > cfg.hbm_budget) throw std::runtime_error("solve footprint exceeds the server's HBM budget");
             } catch (const std::exception& e) {

@@ -164,7 +164,7 @@ void t0_projector(int nb, const cplx* hk, cplx* rho) {
 
 }  // namespace
 
-HostSolve build_solve(const TBModel& m, const SolveParams& p) {
+HostSolve build_solve(const TBModel& m, const SolveParams& p, bool host_state) {
     if (p.nr3 != 1) throw std::runtime_error("nr3 must be 1: the reference propagates 2-D supercells only");
     if (p.nr1 < 4 || p.nr2 < 4 || (p.nr1 % 2) || (p.nr2 % 2))
         throw std::runtime_error("nr1 and nr2 must be even and >= 4 (odd grids break the reference's fftshift pair)");
@@ -532,6 +532,8 @@ HostSolve build_solve(const TBModel& m, const SolveParams& p) {
 
     // Initial state: per k, H_k = sum_s H_s e^{-2 pi i m.n_s/N} and the T=0
     // projector on its negative-energy bands, element-major on the device.
+    // Solves bound for the device may leave it to k_init_state (rho0 empty).
+    if (!host_state) return hs;
     hs.rho0.assign((size_t)2 * nb2 * N, 0.0);
     auto work = [&](int k0, int k1) {
         std::vector<cplx> hk(nb2), rho(nb2);

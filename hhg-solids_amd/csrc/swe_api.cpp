@@ -91,8 +91,13 @@ static void sim_init(SWESim* s, bool device = true) {
     st.num_orb = m.nb;
     st.num_sites = m.nR;
     const hhg::SolveParams p = params_of(st);
-    auto hs = std::make_unique<hhg::HostSolve>(hhg::build_solve(m, p));
+    // device solves compute the initial state on the GPU and keep a host copy
+    auto hs = std::make_unique<hhg::HostSolve>(hhg::build_solve(m, p, !device));
     hhg::DeviceSolve* dev = device ? hhg::dev_create(*hs) : nullptr;
+    if (dev) {
+        hs->rho0.resize((size_t)2 * hs->d.nb * hs->d.nb * hs->d.N);
+        hhg::dev_get_rho(dev, hs->rho0.data());
+    }
     if (dev) hhg::dev_set_variant(dev, s->variant);
     hhg::dev_destroy(s->dev);
     hhg::fftform_destroy(s->ff);

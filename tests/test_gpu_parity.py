@@ -204,3 +204,19 @@ def test_polarisation_with_z_component(oracle_lib, tmp_path, nb):
     err = rel_l2(J, Jref)
     print(f"nb={nb} kernel {kind}: {err:.2e}")
     assert err <= FP64_TOL
+
+
+@pytest.mark.parametrize("nb,nr", [(2, (12, 16)), (4, (10, 12)), (8, (8, 10)), (16, (6, 8))])
+def test_device_initial_state_equals_host(tmp_path, nb, nr):
+    """The initial state of a device solve (k_init_state: H_k, Jacobi, T = 0
+    occupation on the GPU) equals the host setup's, and is a projector."""
+    from hhgamd import lib, tbgen
+    path = str(tmp_path / f"syn{nb}_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=50 + nb))
+    p = make_param(path, nr=nr, tmax=1.0)
+    with lib.Solve(p) as s:
+        rho_dev = s.rho_k()
+    with lib.Solve(p, device=False) as s:
+        rho_host = s.rho_k()
+    assert np.abs(rho_dev - rho_host).max() < 1e-12
+    assert np.abs(np.einsum("kij,kjl->kil", rho_dev, rho_dev) - rho_dev).max() < 1e-12
