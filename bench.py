@@ -88,6 +88,28 @@ def cpu_baseline(path_tb, nr, sample_steps):
                       f"{dt:.1f} s"}
 
 
+def c1_full(path_tb):
+    """The CPU-runnable configuration in full (30x30, 4096 RK4 steps): the GPU
+    plain path and the oracle's FFT-form restatement of the reference loop on
+    one core, both over the whole solve, and their J(t) agreement."""
+    import oracle
+    from hhgamd import lib
+    prm = workload_param(path_tb, 30, "fp64")
+    with lib.Solve(prm) as s:
+        s.run()
+        a = time.perf_counter()
+        s.run()
+        gpu_s = time.perf_counter() - a
+        nk, nt = s.N, s.nt
+        _, Jg = s.current()
+    a = time.perf_counter()
+    _, Jc = oracle.run(path_tb, prm, mode="fft")
+    cpu_s = time.perf_counter() - a
+    return {"workload": "haldane-2band-30x30-nt4096-fp64", "gpu_value": nk * nt / gpu_s, "gpu_ms": gpu_s * 1e3,
+            "cpu_fftform_value": nk * nt / cpu_s, "cpu_seconds": round(cpu_s, 2), "cpu_cores": 1,
+            "j_rel_l2_gpu_vs_cpu": float(np.linalg.norm(Jg - Jc) / np.linalg.norm(Jc))}
+
+
 def cpu_perk_threads(path_tb, nr, steps, threads):
     import oracle
     prm = workload_param(path_tb, nr, "fp64")
@@ -193,6 +215,7 @@ def main():
     if not args.no_cpu:  # CPU leg: the oracle, as baseline and as the FP64 checker of J(t)
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(path_tb, args.nr, args.cpu_steps)
+        out["c1_full"] = c1_full(path_tb)
         out["cpu_perk_threads"] = cpu_perk_threads(path_tb, args.nr, 64, threads)
         if not args.no_check:
             import oracle
