@@ -846,12 +846,14 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // loads every operator block of a support unconditionally (absent blocks are
 // zero in HBM), so its loads issue together instead of one per branch.
 struct SepLayout {
-    size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_k, kbytes, off_jw, bytes;
+    size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_w, off_b, off_k, kbytes, off_jw, bytes;
 };
 // Tables of NBP <= 8 are double-buffered (built one step ahead); NBP = 16
 // keeps one buffer and builds between two barriers.
 __host__ __device__ constexpr int sep_nbuf(int NBP) { return NBP <= 8 ? 2 : 1; }
-__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W) {
+// nstage: support blocks x elements whose H and rbc blocks are staged in LDS
+// (FP32 path, NBP <= 8: ns nb^2), 0 when the table build reads them from HBM.
+__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage) {
     SepLayout L;
     const size_t E = (size_t)NBP * NBP;
     const int GPW = 64 / ((NBP / 2) * (NBP / 2));
@@ -861,7 +863,10 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     L.off_g = o; o = align16(o + nbuf * L.gbytes);
     L.gvbytes = (size_t)MAXD * 3 * E * 16;
     L.off_gv = o; o = align16(o + nbuf * L.gvbytes);
-    L.off_sup = o; o = align16(o + MAX_SUPPORT * 16 + MAX_SUPPORT * 4 + 16 * 4 + (size_t)MAXD * E * 4);
+    L.off_sup = o; o = align16(o + MAX_SUPPORT * 16 + MAX_SUPPORT * 4 + 16 * 4 + (size_t)MAXD * E * 2);
+    if (nbuf == 2) { L.off_w = o; o = align16(o + 2 * (MAX_SUPPORT * 16 + 16 * 8)); }  // weights + field rows
+    else L.off_w = 0;
+    L.off_b = o; o = align16(o + nstage * 4 * 8);
     L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
     L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
     L.off_jw = o; o = align16(o + sizeof(double) * 2 * W * 6);
@@ -957,7 +962,9 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     using TD = Tile<double, NBP>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int W = blockDim.x >> 6;
-    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W);
+    // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
+    constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8;
+    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0);
     const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -977,20 +984,44 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     cx<double>* sw1 = reinterpret_cast<cx<double>*>(smem + L.off_sup);
     int* ss = reinterpret_cast<int*>(smem + L.off_sup + MAX_SUPPORT * 16);
     int* seg = ss + MAX_SUPPORT;
-    int* ent = seg + 16;
+    uint16_t* ent = reinterpret_cast<uint16_t*>(seg + 16);
     const int nent = nd * nbe;
 
     for (size_t i = tid; i < (L.off_sup - L.off_g) / 16; i += bd)
         reinterpret_cast<double2*>(smem + L.off_g)[i] = make_double2(0.0, 0.0);
+    // ss[q]: support index | 1 << 16 when it has a velocity block
     for (int q = tid; q < d.ns; q += bd) {
         const int s = d.sorder[q];
-        ss[q] = s;
+        ss[q] = s | ((d.opmask[s] & (7u << OP_V)) ? 1 << 16 : 0);
         sw1[q] = ld<double>(ip.tw1 + 2 * posmod((long)m1 * d.n1[s], N1));
     }
     for (int q = tid; q <= nd; q += bd) seg[q] = d.dseg[q];
+    float2* bst = reinterpret_cast<float2*>(smem + L.off_b);  // [n2-sorted q][H, rbc x, y, z][nb^2]
+    if constexpr (STAGE) {
+        for (int i = tid; i < d.ns * 4 * nbe; i += bd) {
+            const int q = i / (4 * nbe), r = i - q * 4 * nbe;  // r = op * nbe + el, op 0..3 = H, rbc
+            const double* b = ip.blk + 2 * ((size_t)d.sorder[q] * NOPS * nbe + r);
+            bst[i] = make_float2((float)b[0], (float)b[1]);
+        }
+    }
+    // Double-buffered tables (NBP <= 8): the build of step ti + 1 runs during
+    // step ti, and its per-support weights tw1[m1 n1_s] e^{i A.R_s} (n2-sorted)
+    // and field row are written two steps ahead into slot ti & 1, so the build
+    // reads LDS only (and the velocity blocks). Single buffer (NBP = 16): the
+    // build reads the phases from the step table itself.
+    constexpr bool DB = sep_nbuf(NBP) == 2;
+    cx<double>* wv = reinterpret_cast<cx<double>*>(smem + L.off_w);
+    double* efs = reinterpret_cast<double*>(smem + L.off_w + 2 * MAX_SUPPORT * 16);
+    auto weights = [&](int ti) {
+        const double* st = ip.steptab + (size_t)ti * d.stride;
+        const int sl = ti & 1;
+        for (int q = tid; q < d.ns; q += bd)
+            wv[sl * MAX_SUPPORT + q] = cmul(sw1[q], ld<double>(st + STEP_HEAD + 2 * (ss[q] & 0xffff)));
+        for (int c = tid; c < 9; c += bd) efs[sl * 16 + c] = st[c];
+    };
     for (int q = tid; q < nent; q += bd) {
         const int dd = q / nbe, el = q - dd * nbe;
-        ent[q] = (dd << 16) | (el << 8) | ((el / nb) << 4) | (el - (el / nb) * nb);  // el < 256, i, j < 16
+        ent[q] = (uint16_t)((dd << 8) | ((el / nb) << 4) | (el - (el / nb) * nb));  // dd < 8, i, j < 16
     }
     __syncthreads();
 
@@ -998,24 +1029,38 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         const double* st = ip.steptab + (size_t)ti * d.stride;
         double ef[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) ef[c] = st[c];
+        for (int c = 0; c < 9; ++c) ef[c] = DB ? efs[(ti & 1) * 16 + c] : st[c];
+        const cx<double>* wq = wv + (ti & 1) * MAX_SUPPORT;
         cx<T>* G = reinterpret_cast<cx<T>*>(smem + L.off_g + buf * L.gbytes);
         cx<double>* GV = reinterpret_cast<cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
         for (int q = tid; q < nent; q += bd) {
             const int code = ent[q];
-            const int dd = code >> 16, el = (code >> 8) & 255, ei = (code >> 4) & 15, ej = code & 15;
+            const int dd = code >> 8, ei = (code >> 4) & 15, ej = code & 15, el = ei * nb + ej;
             // sum_s w_s B_s for H, the three rbc axes and the three v axes
             cx<double> a[NOPS];
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) a[o] = {0, 0};
             const int qe = seg[dd + 1];
             for (int qq = seg[dd]; qq < qe; ++qq) {
-                const int s = ss[qq];
-                const double* bs = ip.blk + 2 * ((size_t)s * NOPS * nbe + el);
+                const int sv = ss[qq];
+                const double* bs = ip.blk + 2 * ((size_t)(sv & 0xffff) * NOPS * nbe + el);
                 cx<double> ob[NOPS];
+                if constexpr (STAGE) {
+                    const float2* b = bst + (size_t)qq * 4 * nbe + el;
 #pragma unroll
-                for (int o = 0; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
-                const cx<double> w = cmul(sw1[qq], ld<double>(st + STEP_HEAD + 2 * s));
+                    for (int o = 0; o < 4; ++o) ob[o] = {(double)b[o * nbe].x, (double)b[o * nbe].y};
+                    if (sv >> 16) {
+#pragma unroll
+                        for (int o = OP_V; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
+                    } else {
+#pragma unroll
+                        for (int o = OP_V; o < NOPS; ++o) ob[o] = {0.0, 0.0};
+                    }
+                } else {
+#pragma unroll
+                    for (int o = 0; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
+                }
+                const cx<double> w = DB ? wq[qq] : cmul(sw1[qq], ld<double>(st + STEP_HEAD + 2 * (sv & 0xffff)));
 #pragma unroll
                 for (int o = 0; o < NOPS; ++o) cfma(a[o], w, ob[o]);
             }
@@ -1054,9 +1099,13 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const T half = (T)(0.5 * ip.dt);
     const T full = (T)ip.dt;
     const double pref = 0.1666666666666 * ip.dt;
+    if constexpr (DB) {
+        weights(0);
+        if (d.nt > 1) weights(1);
+        __syncthreads();
+    }
     build(0, 0);
     __syncthreads();
-    constexpr bool DB = sep_nbuf(NBP) == 2;
     for (int ti = 0; ti < d.nt; ++ti) {
         const int buf = DB ? (ti & 1) : 0, jb = ti & 1;
         if (DB && ti + 1 < d.nt) build(ti + 1, buf ^ 1);
@@ -1195,6 +1244,9 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
             for (int c = 0; c < 6; ++c) jw[(jb * W + wave) * 6 + c] = jc[c];
         }
+        // slot ti & 1 was last read by the build of step ti, which finished
+        // before the previous barrier
+        if (DB && ti + 2 < d.nt) weights(ti + 2);
         __syncthreads();
         if (tid < 6) {
             double sacc = 0.0;
@@ -1584,12 +1636,13 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w) {
     if (!(d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep)) return k;
     const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
     const int maxd = d.nd <= 5 ? 5 : 8;
+    const size_t nstage = (fp32 && nbp <= 8) ? (size_t)d.ns * d.nb * d.nb : 0;
     if (w <= 0) {
         const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
-        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w).bytes > LDS_MAX) --w;
+        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage).bytes > LDS_MAX) --w;
     }
-    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w).bytes > LDS_MAX) return k;
+    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage).bytes > LDS_MAX) return k;
     k.kind = KIND_SEP;
     k.a = nbp;
     k.b = w;
@@ -1661,7 +1714,8 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 }
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
-    if (k.kind == KIND_SEP) return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b).bytes;
+    if (k.kind == KIND_SEP)
+        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0).bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = std::min(64, pow2ceil(d.nb * d.nb));
     const int kpb = 256 / G;
