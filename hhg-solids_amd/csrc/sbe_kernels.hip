@@ -316,12 +316,12 @@ __device__ __forceinline__ void thread_per_k_body(const SolveDesc& d, const Inst
 // Hermitian 2x2 matrices H = [a b; b* d] and rho is
 //   K00 = -K11 = 2 Im(b c*),   K01 = -i ((a - d) c + (q - p) b),
 // and the current is real: J = m0 p + m1 q + m2 Re(c) + m3 Im(c).
-// The H coefficients of NREP pairs stay in registers for the whole time
-// loop; the rbc and v coefficients are read from LDS (broadcast reads).
+// The coefficients of the NREP pairs, with this k's twiddle e^{-i k.R} folded
+// in, stay in registers for the whole time loop; a step reads only the uniform
+// pair phases and field components of its row of the step table.
 template <int NREP, int NX, int NV, typename T>
 __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
     const cdptr coef = as_const(ip.pcoef);
-    const cdptr steps = as_const(ip.psteps);
     const double* __restrict__ tw1 = ip.tw1;
     const double* __restrict__ tw2 = ip.tw2;
     const double* __restrict__ rho_in = ip.rho_in;
@@ -340,25 +340,34 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     __shared__ double jl[64 * JS::ROW];
     constexpr int NXA = NX > 0 ? NX : 1;
 
-    // All coefficients in VGPRs for the whole time loop. They are uniform, so
-    // the compiler would hold them in SGPRs and spill; adding 0.0 * lane
-    // (exact for finite values) makes each copy lane-varying.
-    const double lz = 0.0 * (double)lane;
-    cx<double> w0[NREP];
+    // Coefficients in VGPRs for the whole time loop, with this k's twiddle
+    // w0 = e^{-i k.R} folded in: a pair term c0 Re(w) + c1 Im(w), w = w0 phi(t),
+    // equals phi_r (c0 w0r + c1 w0i) + phi_i (c1 w0r - c0 w0i), so a step needs
+    // the uniform phase phi of each pair and no per-k complex product.
+    const double lz = 0.0 * (double)lane;  // lane-varying zero for the uniform origin terms
     T hc[NREP][8], xc[NREP][NXA][8];
     double vc[NREP][NV][8];
 #pragma unroll
     for (int r = 0; r < NREP; ++r) {
         const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
-        w0[r] = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
+        const cx<double> w0 = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
         const cdptr cr = coef + PAIR_HEAD + PAIR_REP * r;
+        auto fold = [&](const double c0, const double c1, double& a, double& b) {
+            a = c0 * w0.re + c1 * w0.im;
+            b = c1 * w0.re - c0 * w0.im;
+        };
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            hc[r][j] = (T)(cr[j] + lz);
+        for (int j = 0; j < 4; ++j) {
+            double a, b;
+            fold(cr[2 * j], cr[2 * j + 1], a, b);
+            hc[r][2 * j] = (T)a; hc[r][2 * j + 1] = (T)b;
 #pragma unroll
-            for (int x = 0; x < NX; ++x) xc[r][x][j] = (T)(cr[8 + 8 * x + j] + lz);
+            for (int x = 0; x < NX; ++x) {
+                fold(cr[8 + 8 * x + 2 * j], cr[8 + 8 * x + 2 * j + 1], a, b);
+                xc[r][x][2 * j] = (T)a; xc[r][x][2 * j + 1] = (T)b;
+            }
 #pragma unroll
-            for (int v = 0; v < NV; ++v) vc[r][v][j] = cr[32 + 8 * v + j] + lz;
+            for (int v = 0; v < NV; ++v) fold(cr[32 + 8 * v + 2 * j], cr[32 + 8 * v + 2 * j + 1], vc[r][v][2 * j], vc[r][v][2 * j + 1]);
         }
     }
     T h0[4], x0[NXA][4];
@@ -383,63 +392,64 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     const T full = (T)dt;
     const double pref = 0.1666666666666 * dt;
 
-    // per-step row, loaded one step ahead
-    double ph[NREP][2], ef[3][NXA];
-    auto load_row = [&](int ti, double (&a)[NREP][2], double (&e)[3][NXA]) {
-        const cdptr row = steps + (size_t)ti * pd.stride;
-#pragma unroll
-        for (int r = 0; r < NREP; ++r) { a[r][0] = row[PAIR_STEP_HEAD + 2 * r]; a[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int x = 0; x < NXA; ++x) e[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
+    // State of a step: its row of the step table (pair phases phi, field) and
+    // the field-free Fourier sums of H and rbc. Two buffers alternate: step ti
+    // reads its own and fills the other one for step ti + 1 -- the row load is
+    // issued at the top of the step and the sums are formed at its end, so the
+    // load has the whole step to arrive. The time loop is unrolled by two so
+    // that the alternation needs no register copies.
+    struct Buf {
+        double ph[NREP][2], ef[3][NXA];
+        T hs[4], xs[NXA][4];
     };
-    load_row(0, ph, ef);
-
-    // Phases and the field-free Fourier sums of a step depend only on the
-    // step table, so they are formed one step ahead, in the same block as the
-    // serial RK4 stage chain of the current step, whose latency they fill.
-    cx<double> w[NREP];
-    T hs[4], xs[NXA][4];
-    auto fourier = [&](const double (&a)[NREP][2], cx<double> (&wo)[NREP], T (&ho)[4], T (&xo)[NXA][4]) {
+    auto sums = [&](Buf& b) {
 #pragma unroll
-        for (int r = 0; r < NREP; ++r) wo[r] = cmul(w0[r], cx<double>{a[r][0], a[r][1]});
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ho[j] = h0[j];
+        for (int j = 0; j < 4; ++j) b.hs[j] = h0[j];
 #pragma unroll
         for (int r = 0; r < NREP; ++r) {
-            const T wr = (T)wo[r].re, wi = (T)wo[r].im;
+            const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) ho[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, ho[j]));
+            for (int j = 0; j < 4; ++j) b.hs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, b.hs[j]));
         }
 #pragma unroll
         for (int x = 0; x < NX; ++x) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xo[x][j] = x0[x][j];
+            for (int j = 0; j < 4; ++j) b.xs[x][j] = x0[x][j];
 #pragma unroll
             for (int r = 0; r < NREP; ++r) {
-                const T wr = (T)wo[r].re, wi = (T)wo[r].im;
+                const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) xo[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, xo[x][j]));
+                for (int j = 0; j < 4; ++j) b.xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, b.xs[x][j]));
             }
         }
     };
-    fourier(ph, w, hs, xs);
+    // Rows arrive as vector loads (lane-varying zero offset; nt is never
+    // negative): scalar loads share their wait counter with the LDS traffic of
+    // the current staging and would be drained with it.
+    const int zl = lane * (int)(d.nt < 0);
+    auto load_row = [&](int ti, Buf& b) {
+        const double* row = ip.psteps + (size_t)ti * pd.stride + zl;
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) { b.ph[r][0] = row[PAIR_STEP_HEAD + 2 * r]; b.ph[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) b.ef[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
+    };
 
-    for (int ti = 0; ti < d.nt; ++ti) {
-        double phn[NREP][2], efn[3][NXA];
-        load_row(ti + 1 < d.nt ? ti + 1 : ti, phn, efn);
-        cx<double> wn[NREP];
-        T hsn[4], xsn[NXA][4];
-        fourier(phn, wn, hsn, xsn);
+    auto step = [&](int ti, const Buf& cur, Buf& nxt) {
+        // the previous chunk of TCH steps is reduced at the top of the step, so
+        // that the stage chain, the current and the next sums form one block
+        if (ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+        load_row(ti + 1 < d.nt ? ti + 1 : ti, nxt);
         T A, D, Br, Bi;
         auto stage_h = [&](int j) {
-            A = hs[0]; D = hs[1]; Br = hs[2]; Bi = hs[3];
+            A = cur.hs[0]; D = cur.hs[1]; Br = cur.hs[2]; Bi = cur.hs[3];
 #pragma unroll
             for (int x = 0; x < NX; ++x) {
-                const T f = (T)ef[j][x];
-                A = fmaT(f, xs[x][0], A); D = fmaT(f, xs[x][1], D);
-                Br = fmaT(f, xs[x][2], Br); Bi = fmaT(f, xs[x][3], Bi);
+                const T f = (T)cur.ef[j][x];
+                A = fmaT(f, cur.xs[x][0], A); D = fmaT(f, cur.xs[x][1], D);
+                Br = fmaT(f, cur.xs[x][2], Br); Bi = fmaT(f, cur.xs[x][3], Bi);
             }
         };
         T kp, kr, ki;
@@ -473,24 +483,36 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
 #pragma unroll
             for (int r = 0; r < NREP; ++r)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) m[j] = fmaT(vc[r][v][2 * j + 1], w[r].im, fmaT(vc[r][v][2 * j], w[r].re, m[j]));
+                for (int j = 0; j < 4; ++j) m[j] = fmaT(vc[r][v][2 * j + 1], cur.ph[r][1], fmaT(vc[r][v][2 * j], cur.ph[r][0], m[j]));
             // inactive lanes hold rho = 0 and contribute exactly 0
             jc[v] = fmaT(m[3], c.im, fmaT(m[2], c.re, fmaT(m[1], q, m[0] * p)));
         }
         JS::put(jl, lane, ti % TCH, jc);
-        if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
-#pragma unroll
-        for (int r = 0; r < NREP; ++r) w[r] = wn[r];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) hs[j] = hsn[j];
-#pragma unroll
-        for (int x = 0; x < NX; ++x)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) xs[x][j] = xsn[x][j];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int x = 0; x < NXA; ++x) ef[j][x] = efn[j][x];
+        sums(nxt);
+    };
+
+    Buf B0, B1;
+    load_row(0, B0);
+    sums(B0);
+    // VGPRs held by the per-k coefficients; the unrolled loop keeps two step
+    // states live and is used while the total stays within the 256 VGPRs
+    constexpr int CREGS = NREP * 8 * (1 + NX) * (int)(sizeof(T) / 4) + NREP * NV * 16;
+    if constexpr (CREGS <= 160) {
+        int ti = 0;
+        for (; ti + 2 <= d.nt; ti += 2) {
+            step(ti, B0, B1);
+            step(ti + 1, B1, B0);
+        }
+        if (ti < d.nt) step(ti, B0, B1);
+    } else {
+        for (int ti = 0; ti < d.nt; ++ti) {
+            step(ti, B0, B1);
+            B0 = B1;
+        }
+    }
+    if (d.nt > 0) {
+        const int t0 = ((d.nt - 1) / TCH) * TCH;  // the last, possibly partial chunk
+        JS::flush(jl, lane, t0, d.nt - t0, jpart, nwaves, wave);
     }
     if (active) {
         double* o = rho_out + 2 * (size_t)k;
@@ -2118,8 +2140,9 @@ void dev_set_variant(DeviceSolve* ds, int variant) {
 int dev_kernel_variant(DeviceSolve* ds, int precision) { return classify(ds, precision).kind; }
 
 // Arithmetic of the launched formulation per k-point and RK4 step, counting
-// a fused multiply-add as 2 flops. Hermitian nb = 2 kernel: pair phases
-// (6 nrep), H and rbc sums (16 nrep each), stage fields (24 nx), four
+// a fused multiply-add as 2 flops. Hermitian nb = 2 kernels: per-k pair phases
+// (6 nrep, streamed form only: the register form folds e^{-i k.R} into its
+// coefficients), H and rbc sums (16 nrep each), stage fields (24 nx), four
 // commutators (4 x 13), stage inputs (24), RK4 accumulation (15), update (8),
 // current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8).
 double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
@@ -2128,7 +2151,8 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     if (kind == KIND_PAIR || kind == KIND_PAIRS) {
         const PairDesc& pd = ds->pd;
         const double jcost = kind == KIND_PAIR ? pd.nv * (16.0 * pd.nrep + 7) : pd.nv * (32.0 * pd.nrep + 14);
-        return 6.0 * pd.nrep + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 + jcost;
+        const double phases = kind == KIND_PAIR ? 0.0 : 6.0 * pd.nrep;
+        return phases + 16.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nx + 52 + 24 + 15 + 8 + jcost;
     }
     int nv = 0, nx = 0;
     for (int a = 0; a < 3; ++a) { nv += (d.vmask >> a) & 1; nx += (d.xmask >> a) & 1; }
