@@ -60,7 +60,11 @@ namespace hhg {
 // Read-only tables with wave-uniform addresses are read through the constant
 // address space, which lets the compiler use scalar (SMEM) loads.
 typedef const __attribute__((address_space(4))) double* cdptr;
+typedef const __attribute__((address_space(4))) float* cfptr;
 __device__ __forceinline__ cdptr as_const(const double* p) { return (cdptr)(p); }
+// Streamed pair coefficients: SC_N doubles, then the same table as floats
+// (read by the FP32 kernels, which would otherwise convert every value).
+constexpr int SC_N = PS_HEAD + PAIR_MAX * PS_REP;
 // An opaque zero: offsets a per-step load so it is not hoisted out of the loop.
 __device__ __forceinline__ int opaque_zero() {
     int z = 0;
@@ -575,6 +579,11 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
     for (int ti = 0; ti < d.nt; ++ti) {
         const cdptr row = steps + (size_t)ti * pd.stride;
         const cdptr cf = as_const(ip.scoef) + opaque_zero();
+        const cfptr cf32 = (cfptr)(as_const(ip.scoef) + SC_N) + opaque_zero();
+        auto cT = [&](int i) -> T {  // H and rbc coefficient i in the stage type
+            if constexpr (std::is_same<T, float>::value) return cf32[i];
+            else return (T)cf[i];
+        };
         cx<double> w[NREP];
 #pragma unroll
         for (int r = 0; r < NREP; ++r) w[r] = cmul(w0[r], ld<double>(row + PAIR_STEP_HEAD + 2 * r));
@@ -582,23 +591,24 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
         double mr[NV][4], mi[NV][4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            hs[j] = (T)cf[j];
+            hs[j] = cT(j);
 #pragma unroll
-            for (int x = 0; x < NX; ++x) xs[x][j] = (T)cf[4 + 4 * x + j];
+            for (int x = 0; x < NX; ++x) xs[x][j] = cT(4 + 4 * x + j);
 #pragma unroll
             for (int v = 0; v < NV; ++v) { mr[v][j] = cf[16 + 8 * v + j]; mi[v][j] = cf[16 + 8 * v + 4 + j]; }
         }
 #pragma unroll
         for (int r = 0; r < NREP; ++r) {
-            const cdptr cr = cf + PS_HEAD + PS_REP * r;
+            const int o = PS_HEAD + PS_REP * r;
+            const cdptr cr = cf + o;
             const T wr = (T)w[r].re, wi = (T)w[r].im;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) hs[j] = fmaT((T)cr[2 * j + 1], wi, fmaT((T)cr[2 * j], wr, hs[j]));
+            for (int j = 0; j < 4; ++j) hs[j] = fmaT(cT(o + 2 * j + 1), wi, fmaT(cT(o + 2 * j), wr, hs[j]));
 #pragma unroll
             for (int x = 0; x < NX; ++x)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    xs[x][j] = fmaT((T)cr[8 + 8 * x + 2 * j + 1], wi, fmaT((T)cr[8 + 8 * x + 2 * j], wr, xs[x][j]));
+                    xs[x][j] = fmaT(cT(o + 8 + 8 * x + 2 * j + 1), wi, fmaT(cT(o + 8 + 8 * x + 2 * j), wr, xs[x][j]));
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const cdptr cv = cr + 32 + 16 * v;
@@ -1550,9 +1560,15 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             HIP_CHECK(hipFreeAsync(scratch, ds->stream));
         }
         ds->pd = hs.pd;
+        std::vector<double> sc;  // alive until the stream synchronisation below
         if (hs.pd.nrep > 0) {
             if (!hs.pcoef.empty()) ds->pcoef = upload(hs.pcoef, ds->stream);
-            ds->scoef = upload(hs.scoef, ds->stream);
+            if (hs.scoef.size() != (size_t)SC_N) throw std::runtime_error("pair coefficient table of unexpected size");
+            sc.assign(SC_N + (SC_N + 1) / 2, 0.0);
+            std::copy(hs.scoef.begin(), hs.scoef.end(), sc.begin());
+            float* f = reinterpret_cast<float*>(sc.data() + SC_N);
+            for (int i = 0; i < SC_N; ++i) f[i] = (float)hs.scoef[i];
+            ds->scoef = upload(sc, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
         }
         ds->rho_tmp = dalloc(sizeof(double) * 2 * (size_t)hs.d.nb * hs.d.nb * hs.d.N, ds->stream);
@@ -1592,7 +1608,7 @@ size_t dev_bytes(const HostSolve& hs) {
     const size_t rho = 2 * (size_t)E * d.N;
     const size_t init = hs.rho0.empty() ? 6 * (size_t)E * d.N : 0;
     return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * rho + init +
-                             hs.pcoef.size() + hs.scoef.size() + hs.psteps.size() + 6 * (size_t)d.nt +
+                             hs.pcoef.size() + hs.scoef.size() * 3 / 2 + 1 + hs.psteps.size() + 6 * (size_t)d.nt +
                              6 * (size_t)d.nt * waves);
 }
 
