@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <map>
@@ -1873,7 +1874,7 @@ static int launch_single(DeviceSolve* ds, int precision, int& nwaves) {
 // batch or a server tunes each shape once.
 static std::mutex g_tune_mu;
 static std::map<std::string, LaunchKey> g_tune_cache;
-static int g_autotune = 0;
+static std::atomic<int> g_autotune{0};  // read by every thread that runs a solve
 
 static std::string shape_of(const DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;

@@ -85,6 +85,11 @@ def set_autotune(on, lib=None):
     bind(lib or load()).hhg_set_autotune(1 if on else 0)
 
 
+def get_autotune(lib=None):
+    """The process-wide tuning switch."""
+    return bool(bind(lib or load()).hhg_get_autotune())
+
+
 class Solve:
     """One solve behind the C ABI, with the extension calls.
 

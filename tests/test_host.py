@@ -182,3 +182,19 @@ def test_server_rejects_bad_policy(native):
         serve.Server(policy="random")
     with pytest.raises(RuntimeError, match="aging"):
         serve.Server(aging_s=0.0)
+
+
+def test_autotune_switch_is_shared_across_threads(native):
+    """The process-wide tuning switch set in one host thread is seen by the others."""
+    import threading
+    seen = []
+    prev = native.get_autotune()
+    native.set_autotune(True)
+    try:
+        t = threading.Thread(target=lambda: seen.append(native.get_autotune()))
+        t.start()
+        t.join(timeout=30)
+        assert seen == [True]
+    finally:
+        native.set_autotune(prev)
+    assert native.get_autotune() == prev
