@@ -220,3 +220,38 @@ def test_device_initial_state_equals_host(tmp_path, nb, nr):
         rho_host = s.rho_k()
     assert np.abs(rho_dev - rho_host).max() < 1e-12
     assert np.abs(np.einsum("kij,kjl->kil", rho_dev, rho_dev) - rho_dev).max() < 1e-12
+
+
+# Short runs: the register-pair kernel steps two at a time with a one-step tail,
+# and every kernel reduces the current in chunks of up to 16 steps with a
+# partial last chunk; nt = 2, 3, 15, 16, 17, 33 covers both boundaries (nt < 2
+# is rejected at init, tests/test_host.py).
+@pytest.mark.parametrize("model", ["haldane", "synthetic2", "synthetic4"])
+@pytest.mark.parametrize("nt", [2, 3, 15, 16, 17, 33])
+def test_short_runs_vs_oracle(native, oracle_lib, tmp_path, model, nt):
+    from hhgamd import tbgen
+    if model == "haldane":
+        path, nr = HALDANE, 12
+    else:
+        nb = int(model[-1])
+        path = str(tmp_path / f"short{nb}_tb.dat")
+        tbgen.write_tb(path, *tbgen.synthetic(nb, seed=40 + nb))
+        nr = (8, 10)
+    dt = 0.02197
+    prm = make_param(path, nr=nr, tmax=dt * nt + 1e-9, dt=dt, tmax_field=dt * nt * 0.9, intensity=4e12,
+                     pol=(0.6, 0.8, 0.0))
+    with native.Solve(prm) as s:
+        assert s.nt == nt
+        s.run()
+        _, J = s.current()
+        kernel = s.kernel()
+    _, J2 = oracle_lib.run(path, prm, mode="perk")
+    err = rel_l2(J, J2)
+    aerr = float(np.abs(J - J2).max())
+    print(f"{model} nt={nt} kernel {kernel}: rel L2 vs oracle = {err:.3e}, max abs {aerr:.1e}, "
+          f"max |J| {np.abs(J2).max():.1e}")
+    assert J.shape == J2.shape == (nt, 3)
+    # Two steps into the pulse J is a near-cancelling k-sum (|J| ~ 1e-9 here, per-k terms ~1e-2), so
+    # its relative error is rounding over a tiny norm; the absolute floor 1e-13 is 1e-10 of the
+    # current's size once the pulse has built up (|J| ~ 1e-3 after three steps).
+    assert err <= FP64_TOL or aerr <= 1e-13

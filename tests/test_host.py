@@ -86,6 +86,13 @@ def test_rejects_bad_grids(native, nr, msg):
         native.Solve(make_param(HMCASE0, nr=nr, tmax=1.0), device=False)
 
 
+@pytest.mark.parametrize("tmax", [0.0, 0.02197 * 1.5])
+def test_rejects_fewer_than_two_steps(native, tmax):
+    """dt is t[1] - t[0] (src/solver.cpp:86-181): a solve needs nt >= 2."""
+    with pytest.raises(RuntimeError, match="at least 2 time steps"):
+        native.Solve(make_param(HMCASE0, nr=8, tmax=tmax, dt=0.02197), device=False)
+
+
 def test_rejects_nr3(native):
     prm = make_param(HMCASE0, nr=8, tmax=1.0)
     prm["nr3"] = 2
