@@ -131,11 +131,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the full-size oracle J check")
-    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "batch", "serve", "fftform"],
+    ap.add_argument("--mode", default="plain", choices=["plain", "precision", "batch", "serve", "fftform", "fft2"],
                     help="plain: the FP64 200x200 solve (default); precision: the 8-band 120x120 solve with FP32 "
                          "stages against FP64; batch: 64 mixed instances in one batched run; serve: Poisson request "
-                         "streams through the continuous-batching server")
+                         "streams through the continuous-batching server; fftform: the reference's real-space "
+                         "algorithm on the GPU; fft2: its LDS radix 2-D FFT alone on device-resident fields")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--fields", type=int, default=8, help="fft2: complex128 fields of nr x nr per transform")
     ap.add_argument("--autotune", action="store_true",
                     help="batch/serve/precision: tune every solve shape's launch configuration on its first run")
     ap.add_argument("--rates", default="0.5,2,5,20", help="serve: arrival rates (requests/s) to sweep")
@@ -158,6 +160,8 @@ def main():
         return bench_precision(args)
     if args.mode == "fftform":
         return bench_fftform(args)
+    if args.mode == "fft2":
+        return bench_fft2(args)
     if args.mode == "serve":
         return bench_serve(args)
 
@@ -320,6 +324,46 @@ def bench_fftform(args):
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": gbs / PEAK_HBM_GBS, "traffic": None,
                         "bytes_per_kstep": bytes_per_kstep, "note": "whole-step streamed bytes over solve time"}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_fft2(args):
+    """The LDS-resident radix 2-D FFT of the FFT form alone, on device-resident
+    data (hhg_fft2_plan_exec on a hipMalloc'd array): by default 2 nb^2 = 8
+    complex128 fields of nr x nr, the forward transform of one RK4 stage of the
+    plain workload (--fields for larger batches).
+    Algorithmic bytes per 2-D transform: two passes (rows, then columns), each
+    reading and writing every field once = 2 * 2 * fields * nr^2 * 16 B; one pass
+    is one k_fft_lines launch."""
+    from hhgamd import lib
+    nch, n = args.fields, args.nr
+    rng = np.random.default_rng(11)
+    a = rng.normal(size=(nch, n, n)) + 1j * rng.normal(size=(nch, n, n))
+    with lib.FFT2Plan(nch, n, n) as plan, lib.DeviceArray((nch, n, n)) as x:
+        x.upload(a)
+        plan.exec(x.address, -1)
+        ref = np.fft.fft2(a)
+        err = float(np.abs(x.download() - ref).max() / np.abs(ref).max())
+        plan.exec(x.address, -1, max(args.warmup, 1))
+        ms = plan.exec(x.address, -1, args.steps)
+    bytes_per_fft = 2 * 2 * nch * n * n * 16
+    gbs = bytes_per_fft * args.steps / (ms * 1e-3) / 1e9
+    flops = 5.0 * nch * n * n * np.log2(n * n)
+    out = {"metric": "LDS radix 2-D FFT throughput (complex128, in place, device-resident)", "value": gbs,
+           "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c128", "data": "synthetic",
+           "config": {"workload": f"fft2-{nch}x{n}x{n}-c128", "fields": nch, "n1": n, "n2": n,
+                      "passes_per_fft": 2},
+           "rel_err_vs_numpy": err,
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_fft": bytes_per_fft,
+                        "gflops_5nlogn": flops * args.steps / (ms * 1e-3) / 1e9}}
+    prof = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(prof):
+        tr = json.load(open(prof)).get(out["config"]["workload"])
+        if tr:  # both passes of one 2-D transform
+            out["roofline"]["traffic"] = tr["bytes_per_launch"]
     print(json.dumps(out), flush=True)
 
 

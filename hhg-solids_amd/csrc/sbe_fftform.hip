@@ -15,8 +15,8 @@
 //  - 1-D transforms of a tile of TB lines are LDS-resident: the workgroup
 //    loads the tile (coalesced along rows, or TB adjacent columns), runs a
 //    mixed-radix Stockham FFT (radices 4, 2, 3, 5, 7, 11, 13) ping-ponging
-//    between two LDS buffers, and stores. Twiddles come from the
-//    e^{-2 pi i j/N} tables of the solve (long-double accurate).
+//    between two LDS buffers, and stores. Twiddles: the e^{-2 pi i j/N}
+//    table of the solve (long-double accurate), staged in LDS with the tile.
 //  - One RK4 step (phase, 4 x [form inputs, FFT rows+cols of 2 nb^2 fields,
 //    commutator, inverse FFT of nb^2 fields], update, window current) is
 //    captured once as a HIP graph and replayed nt times; the kernels read the
@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -41,8 +42,13 @@ namespace fftform {
                                      std::to_string(__LINE__));                                              \
     } while (0)
 
-constexpr int TB = 8;        // lines per workgroup
-constexpr int MAXLEN = 256;  // longest transform (LDS: 2 x TB x MAXLEN complex = 64 KiB)
+// Lines per workgroup of the row pass and of the column pass. Rows are
+// contiguous, so one line per workgroup gives many small workgroups (latency
+// hidden by occupancy); a column tile of 8 adjacent columns reads whole
+// 128-byte row segments.
+constexpr int TB_ROWS = 1;
+constexpr int TB_COLS = 8;
+constexpr int MAXLEN = 256;  // longest transform (LDS: (2 x 8 + 1) x MAXLEN complex = 68 KiB)
 constexpr int MAXPASS = 12;
 
 struct Plan {
@@ -70,22 +76,20 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 twid(const double* tw, int m, int sign) {  // e^{sign 2 pi i m / L}
-    return make_double2(tw[2 * m], sign < 0 ? tw[2 * m + 1] : -tw[2 * m + 1]);
-}
 
 // One radix-R butterfly of the Stockham pass: inputs j + q M, twiddled by
 // e^{sign 2 pi i q k/(Ns R)}, a size-R DFT, outputs (j / Ns) Ns R + k + u Ns.
+// w: the L roots e^{sign 2 pi i m / L}, in LDS.
 template <int R>
-__device__ __forceinline__ void butterfly(const double2* src, double2* dst, const double* tw, int L, int M, int Ns,
-                                          int j, int sign) {
+__device__ __forceinline__ void butterfly(const double2* src, double2* dst, const double2* w, int L, int M, int Ns,
+                                          int j) {
     const int k = j % Ns;
     const int step = L / (Ns * R);
     double2 v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
         const double2 x = src[j + q * M];
-        v[q] = q == 0 ? x : cmul(x, twid(tw, (q * k * step) % L, sign));
+        v[q] = q == 0 ? x : cmul(x, w[(q * k * step) % L]);
     }
     const int base = (j / Ns) * Ns * R + k;
     const int LR = L / R;
@@ -93,7 +97,7 @@ __device__ __forceinline__ void butterfly(const double2* src, double2* dst, cons
     for (int u = 0; u < R; ++u) {
         double2 o = v[0];
 #pragma unroll
-        for (int q = 1; q < R; ++q) o = cadd(o, cmul(v[q], twid(tw, ((q * u) % R) * LR, sign)));
+        for (int q = 1; q < R; ++q) o = cadd(o, cmul(v[q], w[((q * u) % R) * LR]));
         dst[base + u * Ns] = o;
     }
 }
@@ -103,6 +107,7 @@ __device__ __forceinline__ void butterfly(const double2* src, double2* dst, cons
 // workgroup. post: 0 store, 1 store -i x / N (the stage derivative),
 // 2 store x / N.
 // in and out may alias (in-place row pass): the tile is in LDS before any store.
+template <int TB>
 __global__ __launch_bounds__(256) void k_fft_lines(const double2* in, double2* out,
                                                    int nch, int N1, int N2, int dim, int sign,
                                                    const double* __restrict__ tw, Plan plan, int post) {
@@ -113,6 +118,9 @@ __global__ __launch_bounds__(256) void k_fft_lines(const double2* in, double2* o
     const int line0 = blockIdx.x * TB;
     double2* b0 = lds;
     double2* b1 = lds + TB * L;
+    double2* w = lds + 2 * TB * L;  // roots of unity of this length and direction
+    for (int m = threadIdx.x; m < L; m += blockDim.x)
+        w[m] = make_double2(tw[2 * m], sign < 0 ? tw[2 * m + 1] : -tw[2 * m + 1]);
     auto addr = [&](int l, int i) -> size_t {
         const int g = line0 + l, c = g / per_ch, r = g - c * per_ch;
         return dim == 1 ? ((size_t)c * N1 + r) * N2 + i : ((size_t)c * N1 + i) * N2 + r;
@@ -134,13 +142,13 @@ __global__ __launch_bounds__(256) void k_fft_lines(const double2* in, double2* o
             const double2* s = src + l * L;
             double2* d = dst + l * L;
             switch (R) {
-                case 2: butterfly<2>(s, d, tw, L, M, Ns, j, sign); break;
-                case 3: butterfly<3>(s, d, tw, L, M, Ns, j, sign); break;
-                case 4: butterfly<4>(s, d, tw, L, M, Ns, j, sign); break;
-                case 5: butterfly<5>(s, d, tw, L, M, Ns, j, sign); break;
-                case 7: butterfly<7>(s, d, tw, L, M, Ns, j, sign); break;
-                case 11: butterfly<11>(s, d, tw, L, M, Ns, j, sign); break;
-                default: butterfly<13>(s, d, tw, L, M, Ns, j, sign); break;
+                case 2: butterfly<2>(s, d, w, L, M, Ns, j); break;
+                case 3: butterfly<3>(s, d, w, L, M, Ns, j); break;
+                case 4: butterfly<4>(s, d, w, L, M, Ns, j); break;
+                case 5: butterfly<5>(s, d, w, L, M, Ns, j); break;
+                case 7: butterfly<7>(s, d, w, L, M, Ns, j); break;
+                case 11: butterfly<11>(s, d, w, L, M, Ns, j); break;
+                default: butterfly<13>(s, d, w, L, M, Ns, j); break;
             }
         }
         __syncthreads();
@@ -304,13 +312,32 @@ static T* dev_zero(size_t n, hipStream_t s) {
     return dev_upload(z, s);
 }
 
+// One pass of 1-D transforms over all lines of one axis (dim 1: rows of
+// length N2, dim 0: columns of length N1), tb lines per workgroup.
+static void lines_pass(Solve* f, const double2* in, double2* out, int nch, int dim, int sign, int post, int tb) {
+    const int L = dim == 1 ? f->N2 : f->N1;
+    const int lines = nch * (dim == 1 ? f->N1 : f->N2);
+    const int blocks = (lines + tb - 1) / tb;
+    const int threads = tb >= 8 ? 256 : tb >= 2 ? 128 : 64;
+    const size_t lds = sizeof(double2) * (2 * tb + 1) * L;
+    const double* tw = dim == 1 ? f->tw2 : f->tw1;
+    const Plan& p = dim == 1 ? f->p2 : f->p1;
+    switch (tb) {
+        case 1: hipLaunchKernelGGL(k_fft_lines<1>, dim3(blocks), dim3(threads), lds, f->stream, in, out, nch, f->N1,
+                                   f->N2, dim, sign, tw, p, post); break;
+        case 2: hipLaunchKernelGGL(k_fft_lines<2>, dim3(blocks), dim3(threads), lds, f->stream, in, out, nch, f->N1,
+                                   f->N2, dim, sign, tw, p, post); break;
+        case 4: hipLaunchKernelGGL(k_fft_lines<4>, dim3(blocks), dim3(threads), lds, f->stream, in, out, nch, f->N1,
+                                   f->N2, dim, sign, tw, p, post); break;
+        default: hipLaunchKernelGGL(k_fft_lines<8>, dim3(blocks), dim3(threads), lds, f->stream, in, out, nch, f->N1,
+                                    f->N2, dim, sign, tw, p, post); break;
+    }
+}
+
 static void fft2(Solve* f, const double2* in, double2* out, int nch, int sign, int post) {
-    const size_t lds = sizeof(double2) * 2 * TB * std::max(f->N1, f->N2);
-    // rows (length N2), then columns (length N1) with the post-operation
-    hipLaunchKernelGGL(k_fft_lines, dim3((nch * f->N1 + TB - 1) / TB), dim3(256), lds, f->stream, in, (double2*)in,
-                       nch, f->N1, f->N2, 1, sign, f->tw2, f->p2, 0);
-    hipLaunchKernelGGL(k_fft_lines, dim3((nch * f->N2 + TB - 1) / TB), dim3(256), lds, f->stream, in, out, nch,
-                       f->N1, f->N2, 0, sign, f->tw1, f->p1, post);
+    // rows (length N2) in place, then columns (length N1) with the post-operation
+    lines_pass(f, in, (double2*)in, nch, 1, sign, 0, TB_ROWS);
+    lines_pass(f, in, out, nch, 0, sign, post, TB_COLS);
 }
 
 static void destroy(Solve* f) {
@@ -428,39 +455,98 @@ double fftform_run(FFTFormSolve* fs, double* J_out) {
 
 void fftform_destroy(FFTFormSolve* fs) { destroy(reinterpret_cast<Solve*>(fs)); }
 
-void fft2_host(double* data, int nch, int n1, int n2, int sign) {
-    Solve* f = new Solve();
+// e^{-2 pi i j / n}, j < n, rounded from long double
+static std::vector<double> twiddle_table(int n) {
+    std::vector<double> out(2 * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
+        out[2 * j] = (double)cosl(a);
+        out[2 * j + 1] = (double)sinl(a);
+    }
+    return out;
+}
+
+struct FFT2Plan {
+    Solve* f = nullptr;
+    int nch = 0;
+};
+
+void fft2_plan_destroy(FFT2Plan* p) {
+    if (!p) return;
+    destroy(p->f);
+    delete p;
+}
+
+FFT2Plan* fft2_plan_create(int nch, int n1, int n2) {
+    if (nch < 1) throw std::runtime_error("fft2: nch must be positive");
+    FFT2Plan* p = new FFT2Plan();
+    p->nch = nch;
+    p->f = new Solve();
+    Solve* f = p->f;
     try {
         f->N1 = n1; f->N2 = n2; f->N = n1 * n2;
         f->p1 = make_plan(n1);
         f->p2 = make_plan(n2);
-        if (nch < 1) throw std::runtime_error("fft2: nch must be positive");
         FF_CHECK(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
-        auto table = [](int n) {
-            std::vector<double> out(2 * (size_t)n);
-            for (int j = 0; j < n; ++j) {
-                long double a = -2.0L * 3.14159265358979323846264338327950288L * j / n;
-                out[2 * j] = (double)cosl(a);
-                out[2 * j + 1] = (double)sinl(a);
-            }
-            return out;
-        };
-        f->tw1 = dev_upload(table(n1), f->stream);
-        f->tw2 = dev_upload(table(n2), f->stream);
-        const size_t n = (size_t)nch * n1 * n2;
-        std::vector<double2> h(n);
-        for (size_t i = 0; i < n; ++i) h[i] = make_double2(data[2 * i], data[2 * i + 1]);
-        f->F = dev_upload(h, f->stream);
-        fft2(f, f->F, f->F, nch, sign < 0 ? -1 : 1, 0);
-        FF_CHECK(hipGetLastError());
-        FF_CHECK(hipMemcpyAsync(h.data(), f->F, sizeof(double2) * n, hipMemcpyDeviceToHost, f->stream));
+        FF_CHECK(hipEventCreate(&f->e0));
+        FF_CHECK(hipEventCreate(&f->e1));
+        const std::vector<double> t1 = twiddle_table(n1), t2 = twiddle_table(n2);
+        f->tw1 = dev_upload(t1, f->stream);
+        f->tw2 = dev_upload(t2, f->stream);
         FF_CHECK(hipStreamSynchronize(f->stream));
-        for (size_t i = 0; i < n; ++i) { data[2 * i] = h[i].x; data[2 * i + 1] = h[i].y; }
     } catch (...) {
-        destroy(f);
+        fft2_plan_destroy(p);
         throw;
     }
-    destroy(f);
+    return p;
+}
+
+double fft2_plan_exec(FFT2Plan* p, void* data, int sign, int repeats) {
+    if (!p) throw std::runtime_error("fft2: null plan");
+    if (!data) throw std::runtime_error("fft2: null data");
+    if (repeats < 1) throw std::runtime_error("fft2: repeats must be positive");
+    Solve* f = p->f;
+    const size_t bytes = sizeof(double2) * (size_t)p->nch * f->N;
+    if (reinterpret_cast<uintptr_t>(data) % alignof(double2))
+        throw std::runtime_error("fft2: data must be 16-byte aligned complex128");
+    // the kernels touch [data, data + bytes): it must lie in one device allocation
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, data) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        throw std::runtime_error("fft2: data is not device memory");
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    FF_CHECK(hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)data));
+    if ((char*)data + bytes > (char*)base + size)
+        throw std::runtime_error("fft2: the allocation holds fewer than nch * n1 * n2 complex values");
+    double2* d = reinterpret_cast<double2*>(data);
+    FF_CHECK(hipEventRecord(f->e0, f->stream));
+    for (int r = 0; r < repeats; ++r) fft2(f, d, d, p->nch, sign < 0 ? -1 : 1, 0);
+    FF_CHECK(hipGetLastError());
+    FF_CHECK(hipEventRecord(f->e1, f->stream));
+    FF_CHECK(hipStreamSynchronize(f->stream));
+    float ms = 0.f;
+    FF_CHECK(hipEventElapsedTime(&ms, f->e0, f->e1));
+    return ms;
+}
+
+void fft2_host(double* data, int nch, int n1, int n2, int sign) {
+    FFT2Plan* p = fft2_plan_create(nch, n1, n2);
+    double2* dd = nullptr;
+    try {
+        const size_t n = (size_t)nch * n1 * n2;
+        FF_CHECK(hipMalloc((void**)&dd, sizeof(double2) * n));
+        FF_CHECK(hipMemcpy(dd, data, sizeof(double2) * n, hipMemcpyHostToDevice));
+        fft2_plan_exec(p, dd, sign, 1);
+        FF_CHECK(hipMemcpy(data, dd, sizeof(double2) * n, hipMemcpyDeviceToHost));
+    } catch (...) {
+        if (dd) (void)hipFree(dd);
+        fft2_plan_destroy(p);
+        throw;
+    }
+    FF_CHECK(hipFree(dd));
+    fft2_plan_destroy(p);
 }
 
 double fftform_last_ms(FFTFormSolve* fs) { return fs ? reinterpret_cast<Solve*>(fs)->last_ms : 0.0; }

@@ -65,6 +65,13 @@ void fftform_destroy(FFTFormSolve* fs);
 // Unnormalised in-place 2-D FFTs of nch row-major n1 x n2 complex fields on
 // the GPU (host buffer in and out), the transform of the FFT form.
 void fft2_host(double* data, int nch, int n1, int n2, int sign);
+// The same transform on device memory: a plan holds the twiddle tables and a
+// stream; exec transforms the nch fields at data (checked to be one device
+// allocation large enough) `repeats` times and returns the device time (ms).
+struct FFT2Plan;
+FFT2Plan* fft2_plan_create(int nch, int n1, int n2);
+double fft2_plan_exec(FFT2Plan* p, void* data, int sign, int repeats);
+void fft2_plan_destroy(FFT2Plan* p);
 
 // The text dumps of SWESim::test_files (src/swe.cpp:131-181) into test_dir
 // and fields_dir; the r <-> k transforms run on the GPU (fft2_host).

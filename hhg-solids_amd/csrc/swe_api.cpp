@@ -250,6 +250,35 @@ API int hhg_fft2(double* data, int nch, int n1, int n2, int sign) {
         return -1;
     }
 }
+struct hhg_fft2_plan {
+    hhg::FFT2Plan* p = nullptr;
+    ~hhg_fft2_plan() { hhg::fft2_plan_destroy(p); }
+};
+API hhg_fft2_plan* hhg_fft2_plan_create(int nch, int n1, int n2) {
+    try {
+        auto* h = new hhg_fft2_plan();
+        try {
+            h->p = hhg::fft2_plan_create(nch, n1, n2);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        return h;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return nullptr;
+    }
+}
+API double hhg_fft2_plan_exec(hhg_fft2_plan* h, void* data, int sign, int repeats) {
+    try {
+        if (!h) throw std::runtime_error("hhg_fft2_plan_exec: null plan");
+        return hhg::fft2_plan_exec(h->p, data, sign, repeats);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1.0;
+    }
+}
+API void hhg_fft2_plan_destroy(hhg_fft2_plan* h) { delete h; }
 API int SWESim_autotune(SWESim* s, int steps) {
     int kind = -1;
     guarded(s, [&] {

@@ -57,6 +57,8 @@ def bind(lib):
         "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
         "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
         "hhg_fft2": ([np.ctypeslib.ndpointer(dtype=np.complex128, flags="C"), I, I, I, I], I),
+        "hhg_fft2_plan_create": ([I, I, I], V), "hhg_fft2_plan_exec": ([V, V, I, I], Dd),
+        "hhg_fft2_plan_destroy": ([V], None),
         "hhg_batch_new": ([], V), "hhg_batch_delete": ([V], None), "hhg_batch_add": ([V, V], I),
         "hhg_batch_size": ([V], I), "hhg_batch_run": ([V], I), "hhg_batch_last_ms": ([V], Dd),
         "hhg_batch_get_error": ([V], P),
@@ -78,6 +80,98 @@ def fft2(a, sign=-1, lib=None):
     if L.hhg_fft2(x.reshape(-1), nch, n1, n2, int(sign)) != 0:
         raise RuntimeError("hhg_fft2: " + L.hhg_last_error().decode())
     return x
+
+
+class DeviceArray:
+    """A complex128 array in device memory, allocated through the HIP runtime
+    (libamdhip64, the runtime libwannier.so uses): for the entry points that
+    take device addresses. upload/download copy synchronously."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            h = ct.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+            h.hipFree.argtypes = [ct.c_void_p]
+            h.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+            for f in (h.hipMalloc, h.hipFree, h.hipMemcpy):
+                f.restype = ct.c_int
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 16
+        self.ptr = ct.c_void_p()
+        if self.hip().hipMalloc(ct.byref(self.ptr), self.nbytes) != 0:
+            raise RuntimeError(f"hipMalloc of {self.nbytes} bytes failed")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        if self.hip().hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.complex128)
+        if self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:  # device to host
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out
+
+    def close(self):
+        if self.ptr.value:
+            self.hip().hipFree(self.ptr)
+            self.ptr = ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+class FFT2Plan:
+    """The LDS-resident 2-D FFT on device memory: nch row-major n1 x n2
+    complex128 fields, transformed in place. `exec(address, sign, repeats)`
+    takes a device address (DeviceArray.address, or any device allocation whose
+    producers have finished) and returns the device time in milliseconds."""
+
+    def __init__(self, nch, n1, n2, lib=None):
+        self.lib = bind(lib or load())
+        self.shape = (nch, n1, n2)
+        self.ptr = self.lib.hhg_fft2_plan_create(nch, n1, n2)
+        if not self.ptr:
+            raise RuntimeError("hhg_fft2_plan_create: " + self.lib.hhg_last_error().decode())
+
+    def exec(self, device_ptr, sign=-1, repeats=1):
+        ms = self.lib.hhg_fft2_plan_exec(self.ptr, ct.c_void_p(int(device_ptr)), int(sign), int(repeats))
+        if ms < 0:
+            raise RuntimeError("hhg_fft2_plan_exec: " + self.lib.hhg_last_error().decode())
+        return ms
+
+    def close(self):
+        if self.ptr:
+            self.lib.hhg_fft2_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def set_autotune(on, lib=None):

@@ -96,6 +96,18 @@ int hhg_get_autotune(void);
  * 2, 3, 5, 7, 11, 13). The transform of src/fftw_helper.cpp:5-33 without the
  * 1/N of ifft3. Returns 0, or -1 with hhg_last_error set. */
 int hhg_fft2(double* data, int nch, int n1, int n2, int sign);
+/* The same transform on caller-owned device memory (src/fftw_helper.cpp:5-33
+ * fft3/ifft3 on data already resident in HBM). A plan holds the twiddle
+ * tables and its own stream. exec transforms the nch fields at device_data in
+ * place `repeats` times, after checking that device_data is 16-byte aligned
+ * device memory inside one allocation of at least nch*n1*n2 complex128; work
+ * the caller queued on other streams must be complete. Returns the device
+ * time in milliseconds, or -1 with hhg_last_error set. create returns NULL
+ * (hhg_last_error set) for unsupported sizes. */
+typedef struct hhg_fft2_plan hhg_fft2_plan;
+hhg_fft2_plan* hhg_fft2_plan_create(int nch, int n1, int n2);
+double hhg_fft2_plan_exec(hhg_fft2_plan* plan, void* device_data, int sign, int repeats);
+void hhg_fft2_plan_destroy(hhg_fft2_plan* plan);
 /* Arithmetic of the launched formulation per k-point and RK4 step (flops,
  * a fused multiply-add counted as 2). */
 double SWESim_flops_per_kstep(SWESim* swe);

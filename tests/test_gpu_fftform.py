@@ -24,6 +24,51 @@ def test_fft2_matches_numpy(shape, sign):
     assert err <= 1e-13
 
 
+@pytest.mark.parametrize("shape", [(8, 200, 200), (3, 12, 30), (2, 256, 256)])
+def test_fft2_plan_on_device_memory(shape):
+    """hhg_fft2_plan_exec on device memory: forward against numpy, then the
+    unnormalised inverse returns N times the input; repeats chain."""
+    from hhgamd import lib
+    rng = np.random.default_rng(7 + shape[1])
+    a = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    n = shape[1] * shape[2]
+    with lib.FFT2Plan(*shape) as plan, lib.DeviceArray(shape) as x:
+        x.upload(a)
+        ms = plan.exec(x.address, -1)
+        got = x.download()
+        ref = np.fft.fft2(a)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert plan.exec(x.address, +1) > 0
+        err_back = np.abs(x.download() / n - a).max() / np.abs(a).max()
+        # forward twice = N * a(-m) per field (index reversal on both axes)
+        x.upload(a)
+        plan.exec(x.address, -1, repeats=2)
+        rev = np.roll(a[:, ::-1, ::-1], 1, axis=(1, 2)) * n
+        err_twice = np.abs(x.download() - rev).max() / np.abs(rev).max()
+    print(f"{shape}: forward {err:.1e}, inverse {err_back:.1e}, twice {err_twice:.1e}, {ms:.3f} ms")
+    assert err <= 1e-13 and err_back <= 1e-13 and err_twice <= 1e-13
+
+
+def test_fft2_plan_rejects_bad_memory_before_launching():
+    """Host memory, a misaligned address and an allocation smaller than the
+    fields are refused on the host; repeats must be positive."""
+    from hhgamd import lib
+    with lib.FFT2Plan(8, 256, 256) as plan:  # 8 MiB of complex128
+        host = np.zeros((8, 256, 256), dtype=np.complex128)
+        with pytest.raises(RuntimeError, match="not device memory"):
+            plan.exec(host.ctypes.data)
+        with lib.DeviceArray((8, 256, 256)) as x:
+            with pytest.raises(RuntimeError, match="aligned"):
+                plan.exec(x.address + 8)
+            with pytest.raises(RuntimeError, match="repeats"):
+                plan.exec(x.address, repeats=0)
+        with lib.DeviceArray((1 << 16,)) as small:  # 1 MiB
+            with pytest.raises(RuntimeError, match="fewer than"):
+                plan.exec(small.address)
+    with pytest.raises(RuntimeError, match="nch"):
+        lib.FFT2Plan(0, 8, 8)
+
+
 def _run(p, kernel):
     from hhgamd import lib
     with lib.Solve(dict(p, kernel=kernel)) as s:

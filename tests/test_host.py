@@ -161,6 +161,15 @@ def test_fft2_rejects_unsupported_sizes_without_gpu(native):
         native.fft2(np.zeros((8, 512), dtype=complex))
 
 
+def test_fft2_plan_rejects_unsupported_sizes_without_gpu(native):
+    with pytest.raises(RuntimeError, match="factor"):
+        native.FFT2Plan(2, 17, 8)
+    with pytest.raises(RuntimeError, match="at most 256"):
+        native.FFT2Plan(1, 8, 512)
+    with pytest.raises(RuntimeError, match="nch"):
+        native.FFT2Plan(0, 8, 8)
+
+
 def test_library_has_no_undefined_project_symbols(native):
     out = subprocess.run(["nm", "-D", "-C", "--undefined-only", native.LIB_PATH], capture_output=True, text=True,
                          check=True)
