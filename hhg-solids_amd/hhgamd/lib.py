@@ -137,7 +137,10 @@ class DeviceArray:
         self.close()
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the runtime may be gone already
+            pass
 
 
 class FFT2Plan:
@@ -171,7 +174,10 @@ class FFT2Plan:
         self.close()
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may be gone already
+            pass
 
 
 def set_autotune(on, lib=None):
