@@ -14,8 +14,9 @@
 //    become the storage convention and cost nothing.
 //  - 1-D transforms of a tile of TB lines are LDS-resident: the workgroup
 //    loads the tile (coalesced along rows, or TB adjacent columns), runs a
-//    mixed-radix Stockham FFT (radices 4, 2, 3, 5, 7, 11, 13) ping-ponging
-//    between two LDS buffers, and stores. Twiddles: the e^{-2 pi i j/N}
+//    mixed-radix Stockham FFT (radices 4, 2, 3, 5, 7, 11, 13) in one LDS
+//    buffer (each pass holds its butterflies in registers across a barrier),
+//    and stores. Twiddles: the e^{-2 pi i j/N}
 //    table of the solve (long-double accurate), staged in LDS with the tile.
 //  - One RK4 step (phase, 4 x [form inputs, FFT rows+cols of 2 nb^2 fields,
 //    commutator, inverse FFT of nb^2 fields], update, window current) is
@@ -42,13 +43,13 @@ namespace fftform {
                                      std::to_string(__LINE__));                                              \
     } while (0)
 
-// Lines per workgroup of the row pass and of the column pass. Rows are
-// contiguous, so one line per workgroup gives many small workgroups (latency
-// hidden by occupancy); a column tile of 8 adjacent columns reads whole
-// 128-byte row segments.
-constexpr int TB_ROWS = 1;
+// Lines per workgroup of the column pass: 8 adjacent columns read whole
+// 128-byte row segments. Rows are contiguous: a row tile holds about 256
+// points (one line of 200-256, two of up to 128, four of up to 64), which
+// gives many small workgroups without tiny ones.
 constexpr int TB_COLS = 8;
-constexpr int MAXLEN = 256;  // longest transform (LDS: (2 x 8 + 1) x MAXLEN complex = 68 KiB)
+static int rows_per_tile(int L) { return L <= 64 ? 4 : L <= 128 ? 2 : 1; }
+constexpr int MAXLEN = 256;  // longest transform (LDS: (8 + 1) x MAXLEN complex = 36 KiB)
 constexpr int MAXPASS = 12;
 
 struct Plan {
@@ -77,29 +78,57 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-// One radix-R butterfly of the Stockham pass: inputs j + q M, twiddled by
-// e^{sign 2 pi i q k/(Ns R)}, a size-R DFT, outputs (j / Ns) Ns R + k + u Ns.
-// w: the L roots e^{sign 2 pi i m / L}, in LDS.
-template <int R>
-__device__ __forceinline__ void butterfly(const double2* src, double2* dst, const double2* w, int L, int M, int Ns,
-                                          int j) {
-    const int k = j % Ns;
-    const int step = L / (Ns * R);
-    double2 v[R];
+// Threads of a workgroup that transforms TB lines.
+__host__ __device__ constexpr int fft_threads(int TB) { return TB >= 8 ? 256 : TB >= 2 ? 128 : 64; }
+
+// One radix-R pass of the Stockham FFT over the TB lines in buf, in place.
+// Butterfly j of a line reads inputs j + q M (q < R), twiddles them by
+// e^{sign 2 pi i q k/(Ns R)} (k = j mod Ns), forms a size-R DFT and writes
+// outputs (j / Ns) Ns R + k + u Ns. Each thread holds the twiddled inputs of
+// its butterflies (at most NBF) in registers across a barrier, so one LDS
+// buffer suffices. w: the L roots e^{sign 2 pi i m / L}, in LDS.
+template <int R, int TB>
+__device__ __forceinline__ void radix_pass(double2* buf, const double2* w, int L, int Ns) {
+    constexpr int NT = fft_threads(TB);
+    constexpr int NBF = (TB * MAXLEN / R + NT - 1) / NT;
+    const int M = L / R, total = TB * M, step = L / (Ns * R);
+    double2 v[NBF][R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const double2 x = src[j + q * M];
-        v[q] = q == 0 ? x : cmul(x, w[(q * k * step) % L]);
+    for (int b = 0; b < NBF; ++b) {
+        const int t = threadIdx.x + b * NT;
+        if (t < total) {
+            const int l = t / M, j = t - l * M, k = j % Ns;
+            const double2* s = buf + l * L;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const double2 x = s[j + q * M];
+                v[b][q] = q == 0 ? x : cmul(x, w[(q * k * step) % L]);
+            }
+        }
     }
-    const int base = (j / Ns) * Ns * R + k;
-    const int LR = L / R;
+    __syncthreads();
 #pragma unroll
-    for (int u = 0; u < R; ++u) {
-        double2 o = v[0];
+    for (int b = 0; b < NBF; ++b) {
+        const int t = threadIdx.x + b * NT;
+        if (t < total) {
+            const int l = t / M, j = t - l * M, k = j % Ns;
+            double2* d = buf + l * L + (j / Ns) * Ns * R + k;
+            auto output = [&](int u) {
+                double2 a = v[b][0];
 #pragma unroll
-        for (int q = 1; q < R; ++q) o = cadd(o, cmul(v[q], w[((q * u) % R) * LR]));
-        dst[base + u * Ns] = o;
+                for (int q = 1; q < R; ++q) a = cadd(a, cmul(v[b][q], w[((q * u) % R) * M]));
+                d[u * Ns] = a;
+            };
+            if constexpr (R <= 5) {
+#pragma unroll
+                for (int u = 0; u < R; ++u) output(u);
+            } else {  // the large radices stay a loop: registers for the inputs only
+#pragma unroll 1
+                for (int u = 0; u < R; ++u) output(u);
+            }
+        }
     }
+    __syncthreads();
 }
 
 // Transforms of nch fields of N1 x N2 points (row-major, FFT order) along one
@@ -108,17 +137,16 @@ __device__ __forceinline__ void butterfly(const double2* src, double2* dst, cons
 // 2 store x / N.
 // in and out may alias (in-place row pass): the tile is in LDS before any store.
 template <int TB>
-__global__ __launch_bounds__(256) void k_fft_lines(const double2* in, double2* out,
-                                                   int nch, int N1, int N2, int dim, int sign,
-                                                   const double* __restrict__ tw, Plan plan, int post) {
+__global__ __launch_bounds__(fft_threads(TB), TB >= 4 ? 2 : 4) void k_fft_lines(const double2* in, double2* out,
+                                                               int nch, int N1, int N2, int dim, int sign,
+                                                               const double* __restrict__ tw, Plan plan, int post) {
     extern __shared__ double2 lds[];
     const int L = plan.L;
     const int per_ch = dim == 1 ? N1 : N2;
     const int lines = nch * per_ch;
     const int line0 = blockIdx.x * TB;
-    double2* b0 = lds;
-    double2* b1 = lds + TB * L;
-    double2* w = lds + 2 * TB * L;  // roots of unity of this length and direction
+    double2* buf = lds;             // the TB lines
+    double2* w = lds + TB * L;      // roots of unity of this length and direction
     for (int m = threadIdx.x; m < L; m += blockDim.x)
         w[m] = make_double2(tw[2 * m], sign < 0 ? tw[2 * m + 1] : -tw[2 * m + 1]);
     auto addr = [&](int l, int i) -> size_t {
@@ -129,34 +157,24 @@ __global__ __launch_bounds__(256) void k_fft_lines(const double2* in, double2* o
         int l, i;
         if (dim == 1) { l = t / L; i = t - l * L; }
         else { i = t / TB; l = t - i * TB; }
-        if (line0 + l < lines) b0[l * L + i] = in[addr(l, i)];
+        if (line0 + l < lines) buf[l * L + i] = in[addr(l, i)];
     }
     __syncthreads();
-    double2* src = b0;
-    double2* dst = b1;
     int Ns = 1;
     for (int p = 0; p < plan.n; ++p) {
-        const int R = plan.r[p], M = L / R;
-        for (int t = threadIdx.x; t < TB * M; t += blockDim.x) {
-            const int l = t / M, j = t - l * M;
-            const double2* s = src + l * L;
-            double2* d = dst + l * L;
-            switch (R) {
-                case 2: butterfly<2>(s, d, w, L, M, Ns, j); break;
-                case 3: butterfly<3>(s, d, w, L, M, Ns, j); break;
-                case 4: butterfly<4>(s, d, w, L, M, Ns, j); break;
-                case 5: butterfly<5>(s, d, w, L, M, Ns, j); break;
-                case 7: butterfly<7>(s, d, w, L, M, Ns, j); break;
-                case 11: butterfly<11>(s, d, w, L, M, Ns, j); break;
-                default: butterfly<13>(s, d, w, L, M, Ns, j); break;
-            }
+        const int R = plan.r[p];
+        switch (R) {
+            case 2: radix_pass<2, TB>(buf, w, L, Ns); break;
+            case 3: radix_pass<3, TB>(buf, w, L, Ns); break;
+            case 4: radix_pass<4, TB>(buf, w, L, Ns); break;
+            case 5: radix_pass<5, TB>(buf, w, L, Ns); break;
+            case 7: radix_pass<7, TB>(buf, w, L, Ns); break;
+            case 11: radix_pass<11, TB>(buf, w, L, Ns); break;
+            default: radix_pass<13, TB>(buf, w, L, Ns); break;
         }
-        __syncthreads();
-        double2* x = src;
-        src = dst;
-        dst = x;
         Ns *= R;
     }
+    double2* src = buf;
     const double n = (double)N1 * (double)N2;
     for (int t = threadIdx.x; t < TB * L; t += blockDim.x) {
         int l, i;
@@ -318,8 +336,8 @@ static void lines_pass(Solve* f, const double2* in, double2* out, int nch, int d
     const int L = dim == 1 ? f->N2 : f->N1;
     const int lines = nch * (dim == 1 ? f->N1 : f->N2);
     const int blocks = (lines + tb - 1) / tb;
-    const int threads = tb >= 8 ? 256 : tb >= 2 ? 128 : 64;
-    const size_t lds = sizeof(double2) * (2 * tb + 1) * L;
+    const int threads = fft_threads(tb);
+    const size_t lds = sizeof(double2) * (tb + 1) * L;
     const double* tw = dim == 1 ? f->tw2 : f->tw1;
     const Plan& p = dim == 1 ? f->p2 : f->p1;
     switch (tb) {
@@ -336,7 +354,7 @@ static void lines_pass(Solve* f, const double2* in, double2* out, int nch, int d
 
 static void fft2(Solve* f, const double2* in, double2* out, int nch, int sign, int post) {
     // rows (length N2) in place, then columns (length N1) with the post-operation
-    lines_pass(f, in, (double2*)in, nch, 1, sign, 0, TB_ROWS);
+    lines_pass(f, in, (double2*)in, nch, 1, sign, 0, rows_per_tile(f->N2));
     lines_pass(f, in, out, nch, 0, sign, post, TB_COLS);
 }
 
