@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 FP64_TOL = 1e-10
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 10), (3, 12, 30), (2, 200, 240), (2, 14, 26), (1, 256, 256), (4, 60, 4)])
+@pytest.mark.parametrize("shape", [(1, 8, 10), (3, 12, 30), (2, 200, 240), (2, 14, 26), (1, 256, 256), (4, 60, 4),
+                                   (2, 22, 66), (1, 143, 39), (3, 121, 169), (2, 49, 250), (1, 2, 3)])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft2_matches_numpy(shape, sign):
     from hhgamd import lib
