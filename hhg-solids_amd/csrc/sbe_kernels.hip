@@ -28,7 +28,7 @@
 //   k_elem_per_lane<EPL,T>     G = pow2(nb*nb) <= 64 lanes per k, each lane
 //                              owns EPL matrix elements; the stage
 //                              Hamiltonian and stage rho are staged through
-//                              LDS for the commutator. Any model, nb <= 16.
+//                              LDS for the commutator. Any model, nb <= 32.
 //   k_reduce_j                 deterministic sum of the per-wave partials.
 // T is the stage arithmetic (double for the plain path, float for the mixed
 // path); the state rho and the current are always FP64.
@@ -673,13 +673,15 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
 }
 
 // --------------------------------------------------------------------------
-// Element-per-lane kernel for any nb <= 16. A group of G lanes (G = nb*nb
-// rounded up to a power of two, at most 64) owns one k; lane l owns the
-// matrix elements e = l + G*q, q < EPL. Per stage the stage Hamiltonian and
-// stage rho are staged through LDS and each lane forms its commutator
-// elements from row i / column j there. LDS per k: the support phases (ns
-// complex, FP64) and two nb*nb matrices of type T.
-__device__ __forceinline__ int lanes_per_k(int nb) {
+// Element-per-lane kernel for any nb <= 32. A group of G lanes owns one k
+// (G = nb*nb rounded up to a power of two, at most 64, for nb <= 16; the whole
+// 256-lane workgroup for 17 <= nb <= 32); lane l owns the matrix elements
+// e = l + G*q, q < EPL (EPL = 1 for nb <= 8, else 4). Per stage the stage
+// Hamiltonian and stage rho are staged through LDS and each lane forms its
+// commutator elements from row i / column j there. LDS per k: the support
+// phases (ns complex, FP64) and two nb*nb matrices of type T.
+__host__ __device__ inline int lanes_per_k(int nb) {
+    if (nb > 16) return 256;
     int g = 1;
     while (g < nb * nb && g < 64) g <<= 1;
     return g;
@@ -1601,8 +1603,7 @@ size_t dev_bytes(const HostSolve& hs) {
     const SolveDesc& d = hs.d;
     // propagation partials: at most 6 components per wave; elem tiles hold 256/G k in 4 waves
     const int E = d.nb * d.nb;
-    int G = 1;
-    while (G < E && G < 64) G <<= 1;
+    const int G = lanes_per_k(d.nb);
     const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
     // rho and its scratch twin (2 x 2 nb^2 N doubles); a device-side initial
     // state holds 3 more nb^2 N complex planes while it runs
@@ -1611,12 +1612,6 @@ size_t dev_bytes(const HostSolve& hs) {
     return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * rho + init +
                              hs.pcoef.size() + hs.scoef.size() * 3 / 2 + 1 + hs.psteps.size() + 6 * (size_t)d.nt +
                              6 * (size_t)d.nt * waves);
-}
-
-static int pow2ceil(int n) {
-    int g = 1;
-    while (g < n) g <<= 1;
-    return g;
 }
 
 // Which kernel and template instance a solve runs with.
@@ -1696,11 +1691,11 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
     if (!reg_pairs_ok(ds->pd) && (k = key_pairs(ds, fp32)).kind) return k;
     if ((k = key_pair(ds, fp32)).kind) return k;
     if ((k = key_thread(ds, fp32)).kind) return k;
-    if (d.nb > 16) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 16)");
+    if (d.nb > 32) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 32)");
     if ((k = key_sep(ds, fp32, 0)).kind) return k;
     k.kind = KIND_ELEM;
     k.fp32 = fp32;
-    k.a = d.nb * d.nb <= 64 ? 1 : 4;
+    k.a = d.nb <= 8 ? 1 : 4;  // elements per lane (lanes_per_k lanes per k-point)
     return k;
 }
 
@@ -1741,7 +1736,7 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
         return d.N1 * ((d.N2 + kpt - 1) / kpt);
     }
     if (k.kind == KIND_ELEM) {
-        const int G = std::min(64, pow2ceil(d.nb * d.nb));
+        const int G = lanes_per_k(d.nb);
         block = 256;
         waves_per_tile = 4;
         const int kpb = block / G;
@@ -1756,7 +1751,7 @@ static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
     if (k.kind == KIND_SEP)
         return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0).bytes;
     if (k.kind != KIND_ELEM) return 0;
-    const int G = std::min(64, pow2ceil(d.nb * d.nb));
+    const int G = lanes_per_k(d.nb);
     const int kpb = 256 / G;
     const size_t esz = k.fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
     return sizeof(cx<double>) * (size_t)kpb * d.ns + esz * (size_t)kpb * 2 * d.nb * d.nb;

@@ -86,10 +86,12 @@ def test_field_free_is_stationary(native):
 
 
 # band counts: exact tiles (2, 4, 8, 16) and zero-padded ones (3 -> 4, 5 and 6 -> 8, 12 -> 16), one band;
-# nmax = 3 gives 7 distinct n2 values (the deeper per-row tables of the separable kernel)
+# nmax = 3 gives 7 distinct n2 values (the deeper per-row tables of the separable kernel); 20, 24 and 32
+# bands run on the element kernel with a whole workgroup per k-point
 @pytest.mark.parametrize("nb,nr,tmax,nmax", [(1, 10, 10.0, 2), (2, 12, 10.0, 2), (3, 8, 6.0, 2), (4, 10, 10.0, 2),
                                              (5, 8, 6.0, 2), (6, 8, 6.0, 2), (8, 12, 10.0, 2), (12, 6, 3.0, 2),
-                                             (16, 6, 3.0, 2), (4, (8, 10), 6.0, 3)])
+                                             (16, 6, 3.0, 2), (4, (8, 10), 6.0, 3), (20, 6, 2.0, 2), (24, 4, 1.0, 2),
+                                             (32, (4, 6), 1.0, 2)])
 def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, nmax):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
@@ -206,7 +208,8 @@ def test_polarisation_with_z_component(oracle_lib, tmp_path, nb):
     assert err <= FP64_TOL
 
 
-@pytest.mark.parametrize("nb,nr", [(2, (12, 16)), (4, (10, 12)), (8, (8, 10)), (16, (6, 8))])
+@pytest.mark.parametrize("nb,nr", [(2, (12, 16)), (4, (10, 12)), (8, (8, 10)), (16, (6, 8)), (20, (6, 6)),
+                                   (32, (4, 6))])
 def test_device_initial_state_equals_host(tmp_path, nb, nr):
     """The initial state of a device solve (k_init_state: H_k, Jacobi, T = 0
     occupation on the GPU) equals the host setup's, and is a projector."""

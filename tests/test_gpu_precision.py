@@ -51,10 +51,10 @@ def test_fp32_hmcase0_readme():
     assert err <= J_TOL and worst <= HARMONIC_TOL
 
 
-@pytest.mark.parametrize("nb", [2, 4, 8, 16])
-def test_fp32_synthetic(tmp_path, nb):
+@pytest.mark.parametrize("nb,nr", [(2, 24), (4, 24), (8, 24), (16, 24), (20, 12)])
+def test_fp32_synthetic(tmp_path, nb, nr):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
     tbgen.write_tb(path, *tbgen.synthetic(nb, seed=11 + nb))
-    err, worst = check(make_param(path, nr=24, intensity=5e11, pol=(0.6, 0.8, 0.0)))
+    err, worst = check(make_param(path, nr=nr, intensity=5e11, pol=(0.6, 0.8, 0.0)))
     assert err <= J_TOL and worst <= HARMONIC_TOL
