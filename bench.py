@@ -220,7 +220,7 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(path_tb, args.nr, args.cpu_steps)
         out["c1_full"] = c1_full(path_tb)
-        out["cpu_perk_threads"] = cpu_perk_threads(path_tb, args.nr, 64, threads)
+        out["cpu_perk_threads"] = cpu_perk_threads(path_tb, args.nr, 512, threads)
         if not args.no_check:
             import oracle
             _, Jref = oracle.run(path_tb, prm, mode="perk", nthreads=threads)
