@@ -676,10 +676,10 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
 // Element-per-lane kernel for any nb <= 32. A group of G lanes owns one k
 // (G = nb*nb rounded up to a power of two, at most 64, for nb <= 16; the whole
 // 256-lane workgroup for 17 <= nb <= 32); lane l owns the matrix elements
-// e = l + G*q, q < EPL (EPL = 1 for nb <= 8, else 4). Per stage the stage
-// Hamiltonian and stage rho are staged through LDS and each lane forms its
-// commutator elements from row i / column j there. LDS per k: the support
-// phases (ns complex, FP64) and two nb*nb matrices of type T.
+// e = l + G*q, q < EPL (1 for nb <= 8, 4 for 9-16, 2 while nb*nb <= 512, else
+// 4). Per stage the stage Hamiltonian and stage rho are staged through LDS and
+// each lane forms its commutator elements from row i / column j there. LDS per
+// k: the support phases (ns complex, FP64) and two nb*nb matrices of type T.
 __host__ __device__ inline int lanes_per_k(int nb) {
     if (nb > 16) return 256;
     int g = 1;
@@ -1695,7 +1695,8 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
     if ((k = key_sep(ds, fp32, 0)).kind) return k;
     k.kind = KIND_ELEM;
     k.fp32 = fp32;
-    k.a = d.nb <= 8 ? 1 : 4;  // elements per lane (lanes_per_k lanes per k-point)
+    // elements per lane (lanes_per_k lanes per k-point: <= 64 up to 16 bands, 256 beyond)
+    k.a = d.nb <= 8 ? 1 : d.nb <= 16 ? 4 : d.nb * d.nb <= 512 ? 2 : 4;
     return k;
 }
 
@@ -1801,6 +1802,9 @@ static void dispatch(const LaunchKey& k, F&& f) {
             if (k.a == 1)
                 return go(k_elem_per_lane<1, double>, k_elem_per_lane_batch<1, double>, k_elem_per_lane<1, float>,
                           k_elem_per_lane_batch<1, float>);
+            if (k.a == 2)
+                return go(k_elem_per_lane<2, double>, k_elem_per_lane_batch<2, double>, k_elem_per_lane<2, float>,
+                          k_elem_per_lane_batch<2, float>);
             if (k.a == 4)
                 return go(k_elem_per_lane<4, double>, k_elem_per_lane_batch<4, double>, k_elem_per_lane<4, float>,
                           k_elem_per_lane_batch<4, float>);
