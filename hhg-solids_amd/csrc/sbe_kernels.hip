@@ -783,17 +783,41 @@ __device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip
                     lR[ei[q]] = rs[q];
                 }
             __syncthreads();
+            if (d.hermhx) {
+                // Hermitian H and rho: K = -i (M - M^dagger) with M = H rho, one
+                // product; M replaces rho in LDS for the transposed reads
+                cx<T> M[EPL];
 #pragma unroll
-            for (int q = 0; q < EPL; ++q) {
-                cx<T> c = {0, 0};
-                const cx<T>* hrow = lH + eI[q] * NB;
-                const cx<T>* rrow = lR + eI[q] * NB;
+                for (int q = 0; q < EPL; ++q) {
+                    cx<T> c = {0, 0};
+                    const cx<T>* hrow = lH + eI[q] * NB;
 #pragma unroll 4
-                for (int m = 0; m < NB; ++m) {
-                    cmac(c, hrow[m], lR[m * NB + eJ[q]]);
-                    cmsub(c, rrow[m], lH[m * NB + eJ[q]]);
+                    for (int m = 0; m < NB; ++m) cmac(c, hrow[m], lR[m * NB + eJ[q]]);
+                    M[q] = c;
                 }
-                K[q] = {c.im, -c.re};
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < EPL; ++q)
+                    if (l + G * q < E) lR[ei[q]] = M[q];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < EPL; ++q) {
+                    const cx<T> t = lR[eJ[q] * NB + eI[q]];  // M_ji
+                    K[q] = {M[q].im + t.im, t.re - M[q].re};
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < EPL; ++q) {
+                    cx<T> c = {0, 0};
+                    const cx<T>* hrow = lH + eI[q] * NB;
+                    const cx<T>* rrow = lR + eI[q] * NB;
+#pragma unroll 4
+                    for (int m = 0; m < NB; ++m) {
+                        cmac(c, hrow[m], lR[m * NB + eJ[q]]);
+                        cmsub(c, rrow[m], lH[m * NB + eJ[q]]);
+                    }
+                    K[q] = {c.im, -c.re};
+                }
             }
             __syncthreads();
         };
@@ -2160,7 +2184,8 @@ int dev_kernel_variant(DeviceSolve* ds, int precision) { return classify(ds, pre
 // (6 nrep, streamed form only: the register form folds e^{-i k.R} into its
 // coefficients), H and rbc sums (16 nrep each), stage fields (24 nx), four
 // commutators (4 x 13), stage inputs (24), RK4 accumulation (15), update (8),
-// current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8).
+// current (nv (16 nrep + 7)). Generic kernels: 8 (8 nb^3 + 4 ns nb^2 + 16 nv nb^2 + 6 ns / 8), with
+// 4 nb^3 for the element kernel on Hermitian models (one product per stage).
 double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
     const int kind = dev_kernel_variant(ds, precision);
@@ -2186,7 +2211,9 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + 3 * D * (P2 / 4) * 40.0 + table / (kpt * used);
     }
     const double nb = d.nb;
-    return 8.0 * (8 * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;
+    // element kernel: one nb^3 product per stage for Hermitian models, two otherwise
+    const double prods = (kind == KIND_ELEM && d.hermhx) ? 4.0 : 8.0;
+    return 8.0 * (prods * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;
 }
 
 double dev_last_kernel_ms(DeviceSolve* ds) { return ds ? ds->last_kernel_ms : 0.0; }
