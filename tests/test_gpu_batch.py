@@ -66,3 +66,34 @@ def test_batch_vs_oracle_and_continuation(tmp_path, oracle_lib):
         assert rel_l2(J, Jref) <= FP64_TOL
         assert np.abs(s.rho_k() - r1).max() > 0
         s.close()
+
+
+def test_batch_with_element_kernel_members(tmp_path, oracle_lib):
+    """Members on the element kernel (20 and 32 bands: one k-point per
+    256-lane workgroup; a 6-band model forced to the generic kernels) batched
+    with a row-separable one: each equals its solo run and the oracle."""
+    from hhgamd import lib, tbgen
+    files = {}
+    for nb in (6, 20, 32):
+        files[nb] = str(tmp_path / f"big{nb}_tb.dat")
+        tbgen.write_tb(files[nb], *tbgen.synthetic(nb, seed=60 + nb))
+    short = dict(tmax=1.5, tmax_field=1.2, intensity=2e12, pol=(0.6, 0.8, 0.0))
+    params = [make_param(files[20], nr=(6, 6), **short), make_param(files[32], nr=(4, 6), **short),
+              dict(make_param(files[6], nr=(6, 8), **short), kernel="generic"),
+              make_param(files[6], nr=(8, 6), **short)]
+    alone = []
+    for p in params:
+        with lib.Solve(p) as s:
+            s.run()
+            alone.append(s.current()[1])
+    solves = [lib.Solve(p) for p in params]
+    assert [s.kernel() for s in solves] == [3, 3, 3, 6]
+    with lib.Batch(solves) as b:
+        b.run()
+    for p, s, Ja in zip(params, solves, alone):
+        Jb = s.current()[1]
+        _, Jref = oracle_lib.run(p["path_tb"], p, mode="perk")
+        print(f"kernel={s.kernel()} batch vs alone {rel_l2(Jb, Ja):.1e}, vs oracle {rel_l2(Jb, Jref):.1e}")
+        assert rel_l2(Jb, Ja) <= SAME_TOL
+        assert rel_l2(Jb, Jref) <= FP64_TOL
+        s.close()
