@@ -287,9 +287,14 @@ def bench_fftform(args):
     """The reference's real-space algorithm on the GPU (kernel variant 2:
     rho(R), LDS-resident radix FFTs of 3 nb^2 fields per RK4 stage, one step
     replayed as a HIP graph) on the plain workload, against the per-k path.
-    Streamed bytes per k-point and RK4 step: 98 nb^2 complex128 fields touched
-    (per stage: input forming 8, forward FFT rows + columns of 2 nb^2 fields 8,
-    commutator 3, inverse FFT 4; update 6) = 98 * 16 nb^2 bytes."""
+    Streamed complex128 fields per k-point and RK4 step, counting each field
+    read or written once per launch:
+      fused stages (nb <= 4 and a column of all 2 nb^2 fields in LDS): per
+      stage, forming + rows 6 nb^2 + 1 read (+ nb^2 for K_prev after the first
+      stage) and 2 nb^2 written, columns + commutator 2 nb^2 + nb^2, inverse
+      rows nb^2 + nb^2; phase 1, update 6 nb^2 + 1 -> 61 nb^2 + 6 fields;
+      unfused: per stage forming 8, forward FFT rows + columns of 2 nb^2 fields
+      8, commutator 3, inverse FFT 4 (x nb^2), update 6 -> 98 nb^2 fields."""
     from hhgamd import lib, tbgen
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
     path_tb = os.path.join(tmp, "haldane_tb.dat")
@@ -311,14 +316,16 @@ def bench_fftform(args):
     with lib.Solve(prm) as sk:
         sk.run()
         _, Jk = sk.current()
-    bytes_per_kstep = 98 * 16 * nb * nb
+    fused = nb <= 4 and (2 * nb * nb + 2) * args.nr * 16 <= 160 * 1024
+    bytes_per_kstep = 16 * (61 * nb * nb + 6) if fused else 98 * 16 * nb * nb
     ms = wall / args.steps * 1e3
     gbs = bytes_per_kstep * nk * nt / (ms * 1e-3) / 1e9
     out = {"metric": METRIC, "value": nk * nt * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"haldane-2band-{args.nr}x{args.nr}-nt{nt}-fp64-fftform", "nk": nk, "nb": nb,
-                      "nt": nt, "kernel": "FFT form (reference algorithm), graph per RK4 step"},
+                      "nt": nt, "kernel": "FFT form (reference algorithm), graph per RK4 step",
+                      "fused_stages": fused},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "j_rel_l2_vs_per_k": float(np.linalg.norm(Jf - Jk) / np.linalg.norm(Jk)),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",

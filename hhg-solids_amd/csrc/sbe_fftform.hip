@@ -87,9 +87,8 @@ __host__ __device__ constexpr int fft_threads(int TB) { return TB >= 8 ? 256 : T
 // outputs (j / Ns) Ns R + k + u Ns. Each thread holds the twiddled inputs of
 // its butterflies (at most NBF) in registers across a barrier, so one LDS
 // buffer suffices. w: the L roots e^{sign 2 pi i m / L}, in LDS.
-template <int R, int TB>
+template <int R, int TB, int NT>
 __device__ __forceinline__ void radix_pass(double2* buf, const double2* w, int L, int Ns) {
-    constexpr int NT = fft_threads(TB);
     constexpr int NBF = (TB * MAXLEN / R + NT - 1) / NT;
     const int M = L / R, total = TB * M, step = L / (Ns * R);
     double2 v[NBF][R];
@@ -131,6 +130,26 @@ __device__ __forceinline__ void radix_pass(double2* buf, const double2* w, int L
     __syncthreads();
 }
 
+// All radix passes of the plan over the TB lines in buf (NT threads).
+template <int TB, int NT>
+__device__ __forceinline__ void tile_passes(double2* buf, const double2* w, const Plan& plan) {
+    const int L = plan.L;
+    int Ns = 1;
+    for (int p = 0; p < plan.n; ++p) {
+        const int R = plan.r[p];
+        switch (R) {
+            case 2: radix_pass<2, TB, NT>(buf, w, L, Ns); break;
+            case 3: radix_pass<3, TB, NT>(buf, w, L, Ns); break;
+            case 4: radix_pass<4, TB, NT>(buf, w, L, Ns); break;
+            case 5: radix_pass<5, TB, NT>(buf, w, L, Ns); break;
+            case 7: radix_pass<7, TB, NT>(buf, w, L, Ns); break;
+            case 11: radix_pass<11, TB, NT>(buf, w, L, Ns); break;
+            default: radix_pass<13, TB, NT>(buf, w, L, Ns); break;
+        }
+        Ns *= R;
+    }
+}
+
 // Transforms of nch fields of N1 x N2 points (row-major, FFT order) along one
 // axis (dim 1: rows of length N2, dim 0: columns of length N1), TB lines per
 // workgroup. post: 0 store, 1 store -i x / N (the stage derivative),
@@ -160,20 +179,7 @@ __global__ __launch_bounds__(fft_threads(TB), TB >= 4 ? 2 : 4) void k_fft_lines(
         if (line0 + l < lines) buf[l * L + i] = in[addr(l, i)];
     }
     __syncthreads();
-    int Ns = 1;
-    for (int p = 0; p < plan.n; ++p) {
-        const int R = plan.r[p];
-        switch (R) {
-            case 2: radix_pass<2, TB>(buf, w, L, Ns); break;
-            case 3: radix_pass<3, TB>(buf, w, L, Ns); break;
-            case 4: radix_pass<4, TB>(buf, w, L, Ns); break;
-            case 5: radix_pass<5, TB>(buf, w, L, Ns); break;
-            case 7: radix_pass<7, TB>(buf, w, L, Ns); break;
-            case 11: radix_pass<11, TB>(buf, w, L, Ns); break;
-            default: radix_pass<13, TB>(buf, w, L, Ns); break;
-        }
-        Ns *= R;
-    }
+    tile_passes<TB, fft_threads(TB)>(buf, w, plan);
     double2* src = buf;
     const double n = (double)N1 * (double)N2;
     for (int t = threadIdx.x; t < TB * L; t += blockDim.x) {
@@ -251,6 +257,115 @@ __global__ void k_comm(int nb, int N, const double2* __restrict__ F, double2* __
             }
             C[(size_t)(i * nb + j) * N + k] = cv;
         }
+}
+
+// Fused RK4 stage for nb <= 4, part 1: the row pass of the forward FFTs of the
+// 2 nb^2 fields, forming each row as it is loaded (k_form: p(R) (Heff) for the
+// first nb^2 channels, the stage rho + coef K_prev for the others).
+template <int TB>
+__global__ __launch_bounds__(fft_threads(TB), TB >= 4 ? 2 : 4) void k_rows_form(
+    const int* __restrict__ step, const double* __restrict__ steptab, int stride, int js, double coef, int nb2,
+    int N1, int N2, const double2* __restrict__ p, const double2* __restrict__ Hd, const double2* __restrict__ Xd,
+    const double2* __restrict__ rho, const double2* __restrict__ Kprev, const double* __restrict__ tw, Plan plan,
+    double2* __restrict__ F) {
+    extern __shared__ double2 lds[];
+    const int L = plan.L;  // = N2
+    const int N = N1 * N2;
+    const size_t plane = (size_t)nb2 * N;
+    const int lines = 2 * nb2 * N1;
+    const int line0 = blockIdx.x * TB;
+    double2* buf = lds;
+    double2* w = lds + TB * L;
+    for (int m = threadIdx.x; m < L; m += blockDim.x) w[m] = make_double2(tw[2 * m], tw[2 * m + 1]);
+    const double* st = steptab + (size_t)(*step) * stride + 3 * js;
+    for (int t = threadIdx.x; t < TB * L; t += blockDim.x) {
+        const int l = t / L, i = t - l * L, g = line0 + l;
+        if (g >= lines) continue;
+        const int c = g / N1, r = g - c * N1;
+        const size_t j = (size_t)r * N2 + i;
+        double2 v;
+        if (c < nb2) {
+            const size_t o = (size_t)c * N + j;
+            double2 h = Hd[o];
+            const double2 x0 = Xd[o], x1 = Xd[plane + o], x2 = Xd[2 * plane + o];
+            h = make_double2(h.x + st[0] * x0.x, h.y + st[0] * x0.y);
+            h = make_double2(h.x + st[1] * x1.x, h.y + st[1] * x1.y);
+            h = make_double2(h.x + st[2] * x2.x, h.y + st[2] * x2.y);
+            v = cmul(p[j], h);
+        } else {
+            const size_t o = (size_t)(c - nb2) * N + j;
+            const double2 rr = rho[o];
+            if (Kprev) {
+                const double2 k = Kprev[o];
+                v = make_double2(rr.x + coef * k.x, rr.y + coef * k.y);
+            } else {
+                v = rr;
+            }
+        }
+        buf[l * L + i] = v;
+    }
+    __syncthreads();
+    tile_passes<TB, fft_threads(TB)>(buf, w, plan);
+    for (int t = threadIdx.x; t < TB * L; t += blockDim.x) {
+        const int l = t / L, i = t - l * L, g = line0 + l;
+        if (g >= lines) continue;
+        const int c = g / N1, r = g - c * N1;
+        F[(size_t)c * N + (size_t)r * N2 + i] = buf[l * L + i];
+    }
+}
+
+// Fused RK4 stage, part 2: for one column n2 of every field, in LDS: the
+// forward column FFTs of the 2 nb^2 fields, the commutator C_k = [H_k, X_k] at
+// each k of the column (k_comm, src/solver.cpp:1461-1476), and the inverse
+// column FFTs of the nb^2 fields of C. The inverse row pass follows.
+template <int NB>
+__global__ __launch_bounds__(256) void k_cols_comm(int N1, int N2, const double* __restrict__ tw, Plan plan,
+                                                   const double2* __restrict__ F, double2* __restrict__ C) {
+    constexpr int NB2 = NB * NB, NCH = 2 * NB2;
+    extern __shared__ double2 lds[];
+    const int L = plan.L;  // = N1
+    const int N = N1 * N2;
+    const int col = blockIdx.x;
+    double2* buf = lds;           // NCH columns of length L
+    double2* wf = lds + NCH * L;  // e^{-2 pi i m/L}
+    double2* wb = wf + L;         // e^{+2 pi i m/L}
+    for (int m = threadIdx.x; m < L; m += blockDim.x) {
+        wf[m] = make_double2(tw[2 * m], tw[2 * m + 1]);
+        wb[m] = make_double2(tw[2 * m], -tw[2 * m + 1]);
+    }
+    for (int t = threadIdx.x; t < NCH * L; t += blockDim.x) {
+        const int c = t / L, r = t - c * L;
+        buf[t] = F[(size_t)c * N + (size_t)r * N2 + col];
+    }
+    __syncthreads();
+    tile_passes<NCH, 256>(buf, wf, plan);
+    for (int r = threadIdx.x; r < L; r += blockDim.x) {  // one k per thread; C over the H columns
+        double2 h[NB2], x[NB2];
+#pragma unroll
+        for (int e = 0; e < NB2; ++e) {
+            h[e] = buf[e * L + r];
+            x[e] = buf[(NB2 + e) * L + r];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                double2 cv = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int l = 0; l < NB; ++l) {
+                    const double2 a = cmul(h[i * NB + l], x[l * NB + j]);
+                    const double2 b = cmul(x[i * NB + l], h[l * NB + j]);
+                    cv = make_double2(cv.x + (a.x - b.x), cv.y + (a.y - b.y));
+                }
+                buf[(i * NB + j) * L + r] = cv;
+            }
+    }
+    __syncthreads();
+    tile_passes<NB2, 256>(buf, wb, plan);
+    for (int t = threadIdx.x; t < NB2 * L; t += blockDim.x) {
+        const int c = t / L, r = t - c * L;
+        C[(size_t)c * N + (size_t)r * N2 + col] = buf[t];
+    }
 }
 
 // rho += (0.1666666666666 dt)(K0 + 2 K1 + 2 K2 + K3); then the observable
@@ -358,6 +473,42 @@ static void fft2(Solve* f, const double2* in, double2* out, int nch, int sign, i
     lines_pass(f, in, out, nch, 0, sign, post, TB_COLS);
 }
 
+// One RK4 stage in three launches (nb <= 4, a column of all 2 nb^2 fields in
+// LDS): forming + forward row FFTs, forward column FFTs + commutator + inverse
+// column FFTs, inverse row FFTs with the -i/N of the stage derivative.
+static bool fused_stage_ok(int nb, int N1) {
+    return nb <= 4 && sizeof(double2) * (size_t)(2 * nb * nb + 2) * N1 <= 160 * 1024;
+}
+
+static void fused_stage(Solve* f, int stride, int js, double coef, const double2* Kprev, double2* Kout) {
+    const int nb2 = f->nb * f->nb;
+    const int tb = rows_per_tile(f->N2);
+    const int lines = 2 * nb2 * f->N1;
+    const size_t lds_rows = sizeof(double2) * (tb + 1) * f->N2;
+    auto rows = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((lines + tb - 1) / tb), dim3(fft_threads(tb)), lds_rows, f->stream, f->step,
+                           f->steptab, stride, js, coef, nb2, f->N1, f->N2, f->ph, f->Hd, f->Xd, f->rho, Kprev,
+                           f->tw2, f->p2, f->F);
+    };
+    switch (tb) {
+        case 1: rows(k_rows_form<1>); break;
+        case 2: rows(k_rows_form<2>); break;
+        default: rows(k_rows_form<4>); break;
+    }
+    const size_t lds_cols = sizeof(double2) * (size_t)(2 * nb2 + 2) * f->N1;
+    auto cols = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(f->N2), dim3(256), lds_cols, f->stream, f->N1, f->N2, f->tw1, f->p1, f->F,
+                           f->C);
+    };
+    switch (f->nb) {
+        case 1: cols(k_cols_comm<1>); break;
+        case 2: cols(k_cols_comm<2>); break;
+        case 3: cols(k_cols_comm<3>); break;
+        default: cols(k_cols_comm<4>); break;
+    }
+    lines_pass(f, f->C, Kout, nb2, 1, +1, 1, rows_per_tile(f->N2));
+}
+
 static void destroy(Solve* f) {
     if (!f) return;
     if (f->exec) (void)hipGraphExecDestroy(f->exec);
@@ -438,10 +589,15 @@ FFTFormSolve* fftform_create(const HostSolve& hs) {
                            d.N2, f->lat[0], f->lat[1], f->lat[2], f->lat[3], f->lat[4], f->lat[5], f->ph);
         const double coef[4] = {0.0, 0.5 * hs.dt, 0.5 * hs.dt, hs.dt};
         const int field[4] = {0, 1, 1, 2};
+        const bool fused = fused_stage_ok(d.nb, d.N1);
         for (int s = 0; s < 4; ++s) {
+            const double2* Kprev = s == 0 ? (const double2*)nullptr : f->K + (size_t)(s - 1) * plane;
+            if (fused) {
+                fused_stage(f, d.stride, field[s], coef[s], Kprev, f->K + (size_t)s * plane);
+                continue;
+            }
             hipLaunchKernelGGL(k_form, dim3(gp), dim3(blk), 0, f->stream, f->step, f->steptab, d.stride, field[s],
-                               coef[s], nb2, N, f->ph, f->Hd, f->Xd, f->rho,
-                               s == 0 ? (const double2*)nullptr : f->K + (size_t)(s - 1) * plane, f->F);
+                               coef[s], nb2, N, f->ph, f->Hd, f->Xd, f->rho, Kprev, f->F);
             fft2(f, f->F, f->F, 2 * nb2, -1, 0);
             hipLaunchKernelGGL(k_comm, dim3((N + blk - 1) / blk), dim3(blk), 0, f->stream, d.nb, N, f->F, f->C);
             fft2(f, f->C, f->K + (size_t)s * plane, nb2, +1, 1);

@@ -90,10 +90,15 @@ def test_fftform_matches_oracle_fft_form(oracle_lib, path, nr, pol):
 
 
 def test_fftform_matches_per_k_kernels(tmp_path):
+    """nb <= 4 runs each RK4 stage as three fused launches (forming + rows,
+    columns + commutator + inverse columns, inverse rows); nb = 5 takes the
+    unfused path (a column of all 2 nb^2 fields no longer fits in LDS)."""
     from hhgamd import tbgen
-    path = str(tmp_path / "syn3_tb.dat")
-    tbgen.write_tb(path, *tbgen.synthetic(3, seed=5))
-    for pth, nr in [(HALDANE, (24, 30)), (path, (10, 14))]:
+    paths = {}
+    for nb in (3, 4, 5):
+        paths[nb] = str(tmp_path / f"syn{nb}_tb.dat")
+        tbgen.write_tb(paths[nb], *tbgen.synthetic(nb, seed=5 + nb))
+    for pth, nr in [(HALDANE, (24, 30)), (paths[3], (10, 14)), (paths[4], (12, 10)), (paths[5], (8, 10))]:
         p = make_param(pth, nr=nr, tmax=12.0, tmax_field=10.0, pol=(0.8, 0.6, 0.0), phi=(0.3, 0.0, 0.0),
                        intensity=3e12)
         _, Jf = _run(p, "fftform")
