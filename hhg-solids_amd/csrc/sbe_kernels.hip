@@ -934,6 +934,7 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Orders the LDS accesses of one wave: writes before the barrier are seen by
 // every lane's reads after it (LDS executes a wave's instructions in order).
@@ -1194,27 +1195,36 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             TT::st2(bufX, i0 + 1, j0, rs[2], rs[3]);
             wave_lds_sync();
             cx<T> M[4];
-            if constexpr (NBP == 16 && std::is_same<T, float>::value) {
+            if constexpr (NBP == 16) {
                 // 16 x 16 complex product on the matrix cores, one k-point per
                 // wave: Re M = Hr Xr - Hi Xi, Im M = Hr Xi + Hi Xr as four real
-                // 16x16x16 products of four v_mfma_f32_16x16x4f32 each. Lane l
+                // 16x16x16 products of four v_mfma_*_16x16x4 each. Lane l
                 // supplies A[l%16][kc4 + l/16] = H and B[kc4 + l/16][l%16] = X and
-                // receives D[4 (l/16) + r][l%16], r = 0..3.
+                // receives, r = 0..3, D[4 (l/16) + r][l%16] (f32) or
+                // D[4 r + l/16][l%16] (f64).
                 const int ai = lane & 15, kq = lane >> 4;
-                f32x4 accR = {0.f, 0.f, 0.f, 0.f}, accI = {0.f, 0.f, 0.f, 0.f};
+                using acc_t = typename std::conditional<std::is_same<T, float>::value, f32x4, f64x4>::type;
+                acc_t accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+                auto mfma = [](T a, T b, acc_t c) -> acc_t {
+                    if constexpr (std::is_same<T, float>::value) return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+                    else return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+                };
 #pragma unroll
                 for (int kc = 0; kc < 4; ++kc) {
                     const int kk = 4 * kc + kq;
-                    const cx<float> hc = bufH[TT::off(kk, ai)];  // H[ai][kk] = conj(H[kk][ai])
-                    const cx<float> x = bufX[TT::off(kk, ai)];   // X[kk][ai]
-                    accR = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.re, x.re, accR, 0, 0, 0);
-                    accR = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.im, x.im, accR, 0, 0, 0);   // -Hi Xi, Hi = -hc.im
-                    accI = __builtin_amdgcn_mfma_f32_16x16x4f32(hc.re, x.im, accI, 0, 0, 0);
-                    accI = __builtin_amdgcn_mfma_f32_16x16x4f32(-hc.im, x.re, accI, 0, 0, 0);  // Hi Xr
+                    const cx<T> hc = bufH[TT::off(kk, ai)];  // H[ai][kk] = conj(H[kk][ai])
+                    const cx<T> x = bufX[TT::off(kk, ai)];   // X[kk][ai]
+                    accR = mfma(hc.re, x.re, accR);
+                    accR = mfma(hc.im, x.im, accR);   // -Hi Xi, Hi = -hc.im
+                    accI = mfma(hc.re, x.im, accI);
+                    accI = mfma(-hc.im, x.re, accI);  // Hi Xr
                 }
                 wave_lds_sync();
 #pragma unroll
-                for (int r = 0; r < 4; ++r) bufH[TT::off(4 * kq + r, ai)] = cx<float>{accR[r], accI[r]};
+                for (int r = 0; r < 4; ++r) {
+                    const int row = std::is_same<T, float>::value ? 4 * kq + r : 4 * r + kq;
+                    bufH[TT::off(row, ai)] = cx<T>{accR[r], accI[r]};
+                }
                 wave_lds_sync();
                 TT::ld2(bufH, i0, j0, M[0], M[1]);
                 TT::ld2(bufH, i0 + 1, j0, M[2], M[3]);
