@@ -21,14 +21,15 @@
 //   k_sep<NBP,MAXD,T>          Hermitian 3 <= nb <= 16: row-separable Fourier
 //                              sums built once per k-row and step in LDS,
 //                              2x2 element blocks per lane, M - M^dagger
-//                              commutator.
+//                              commutator (16 bands: M on the matrix cores).
 //   k_thread_per_k<NB,MAXU,T>  one lane per k, everything in registers; the
 //                              blocks and the per-step table are uniform and
 //                              arrive through scalar loads. For nb <= 2.
-//   k_elem_per_lane<EPL,T>     G = pow2(nb*nb) <= 64 lanes per k, each lane
-//                              owns EPL matrix elements; the stage
-//                              Hamiltonian and stage rho are staged through
-//                              LDS for the commutator. Any model, nb <= 32.
+//   k_elem_per_lane<EPL,T>     G = pow2(nb*nb) <= 64 lanes per k (a whole
+//                              workgroup beyond 16 bands), each lane owns EPL
+//                              matrix elements; the stage Hamiltonian and
+//                              stage rho are staged through LDS for the
+//                              commutator. Any model, nb <= 32.
 //   k_reduce_j                 deterministic sum of the per-wave partials.
 // T is the stage arithmetic (double for the plain path, float for the mixed
 // path); the state rho and the current are always FP64.
@@ -901,9 +902,10 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // Each lane owns a 2x2 element block, so every LDS read of the product feeds
 // two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and exchange H,
 // X and M through LDS with wave-level ordering only. One workgroup barrier per
-// step: the tables of step t+1 are built while step t runs. The table build
-// loads every operator block of a support unconditionally (absent blocks are
-// zero in HBM), so its loads issue together instead of one per branch.
+// step: the tables of step t+1 are built while step t runs (NBP <= 8). The
+// build reads the per-support weights and field row from LDS (written two
+// steps ahead) and, on the FP32 path, the H and rbc blocks staged in LDS at
+// launch; only the velocity blocks of the window supports come from HBM.
 struct SepLayout {
     size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_w, off_b, off_k, kbytes, off_jw, bytes;
 };
