@@ -442,7 +442,9 @@ def bench_batch(args):
 def bench_serve(args):
     """Poisson streams of mixed solves (nb in {2,4,8,16}, 5-smooth grids in
     [48, 240], tmax U[60, 150]) through the server at each arrival rate;
-    reports p50/p99 latency (arrival to J available) and k-point*RK4-steps/s."""
+    reports p50/p99 latency (arrival to J available) and k-point*RK4-steps/s.
+    Every rate replays the same requests (one seed; the exponential gaps scale
+    with 1/rate), so the sweep compares like with like."""
     from hhgamd import serve, workloads
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
     rates = [float(r) for r in args.rates.split(",")]
@@ -450,17 +452,17 @@ def bench_serve(args):
     with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8, policy=args.policy) as srv:
         warm = workloads.c5(tmp, 2, rate=100.0, seed=99, precision=args.precision)
         serve.run_stream(srv, [(0.0, p) for _, p in warm])
-        for i, rate in enumerate(rates):
-            arrivals = workloads.c5(tmp, args.requests, rate=rate, seed=50 + i, precision=args.precision)
+        for rate in rates:
+            arrivals = workloads.c5(tmp, args.requests, rate=rate, seed=50, precision=args.precision)
             _, summ = serve.run_stream(srv, arrivals)
             summ["rate_per_s"] = rate
             sweep.append(summ)
             print(json.dumps({"serve_rate": rate, **summ}), file=sys.stderr, flush=True)
-    # the requests of the highest-rate stream one by one on the plain FP64 path
+    # the same requests one by one on the plain FP64 path
     from hhgamd import lib
     top = len(rates) - 1
     plain_s, plain_ksteps = 0.0, 0
-    for _, p in workloads.c5(tmp, args.requests, rate=rates[top], seed=50 + top, precision="fp64"):
+    for _, p in workloads.c5(tmp, args.requests, rate=rates[top], seed=50, precision="fp64"):
         with lib.Solve(p) as s:
             a = time.perf_counter()
             s.run()
