@@ -106,6 +106,7 @@ void set_autotune(int on);
 int get_autotune();
 // Launch key the next run uses: kind, a, b, c, fp32.
 void dev_launch_key(DeviceSolve* ds, int precision, int* out5);
+int dev_kpoints_per_wave(DeviceSolve* ds, int precision);
 // Flops per k-point and RK4 step of the formulation the next run launches.
 double dev_flops_per_kstep(DeviceSolve* ds, int precision);
 void dev_destroy(DeviceSolve* ds);

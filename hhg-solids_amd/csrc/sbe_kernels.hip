@@ -907,7 +907,7 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // steps ahead) and, on the FP32 path, the H and rbc blocks staged in LDS at
 // launch; only the velocity blocks of the window supports come from HBM.
 struct SepLayout {
-    size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_w, off_b, off_k, kbytes, off_jw, bytes;
+    size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_w, off_b, off_k, kbytes, off_z, off_jw, bytes;
 };
 // Tables of NBP <= 8 are double-buffered (built one step ahead); NBP = 16
 // keeps one buffer and builds between two barriers.
@@ -930,6 +930,7 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     L.off_b = o; o = align16(o + nstage * 4 * 8);
     L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
     L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
+    L.off_z = o; o = align16(o + (NBP == 16 ? (size_t)W * 4 * MAXD * 16 : 0));  // row twiddles, 4 k-points per wave
     L.off_jw = o; o = align16(o + sizeof(double) * 2 * W * 6);
     L.bytes = o;
     return L;
@@ -1032,12 +1033,14 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / LPK, bl = lane - g * LPK;
     const int i0 = 2 * (bl / HB), j0 = 2 * (bl - (bl / HB) * HB);
-    const int kpt = W * GPW;
+    // KPW k-points per wave (NBP = 16: ip.kpw = 1, 2 or 4) share each table
+    // build; k-point p of wave w sits at m2 = base + (p W + w) GPW + g
+    constexpr int KPW_MAX = NBP == 16 ? 4 : 1;
+    const int KPW = KPW_MAX > 1 ? ip.kpw : 1;
+    const int kpt = W * GPW * KPW;
     const int tpr = (N2 + kpt - 1) / kpt;
     const int m1 = tile / tpr;
-    const int m2 = (tile - m1 * tpr) * kpt + wave * GPW + g;
-    const bool kval = m2 < N2;
-    const int k = m1 * N2 + (kval ? m2 : 0);
+    auto m2_of = [&](int p) { return (tile - m1 * tpr) * kpt + (p * W + wave) * GPW + g; };
     cx<T>* bufH = reinterpret_cast<cx<T>*>(smem + L.off_k + (size_t)(wave * GPW + g) * L.kbytes);  // H, then M
     cx<T>* bufX = bufH + E;
     double* jw = reinterpret_cast<double*>(smem + L.off_jw);
@@ -1142,17 +1145,40 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         }
     };
 
-    cx<double> rho[4];
+    cx<double> rhoA[KPW_MAX][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
-        rho[q] = (kval && ei < nb && ej < nb) ? ld<double>(ip.rho_in + 2 * ((size_t)(ei * nb + ej) * N + k))
-                                              : cx<double>{0.0, 0.0};
+    for (int p = 0; p < KPW_MAX; ++p) {
+        const int m2 = m2_of(p);
+        const bool kval = p < KPW && m2 < N2;
+        const int k = m1 * N2 + (kval ? m2 : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+            rhoA[p][q] = (kval && ei < nb && ej < nb) ? ld<double>(ip.rho_in + 2 * ((size_t)(ei * nb + ej) * N + k))
+                                                      : cx<double>{0.0, 0.0};
+        }
     }
-    cx<double> z[MAXD];
+    // row twiddles z_d = tw2[m2 n2_d] (0 for k-points past the row end): in
+    // registers for one k-point per wave, else per wave in LDS (wave-uniform
+    // for NBP = 16, read as broadcasts)
+    cx<double> z[KPW_MAX > 1 ? 1 : MAXD];
+    cx<double>* zt = reinterpret_cast<cx<double>*>(smem + L.off_z) + (size_t)wave * KPW_MAX * MAXD;
+    if constexpr (KPW_MAX > 1) {
+        if (lane < KPW_MAX * MAXD) {
+            const int p = lane / MAXD, dd = lane - p * MAXD, m2 = m2_of(p);
+            zt[lane] = (p < KPW && dd < nd && m2 < N2) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2))
+                                                       : cx<double>{0.0, 0.0};
+        }
+    } else {
+        const int m2 = m2_of(0);
 #pragma unroll
-    for (int dd = 0; dd < MAXD; ++dd)
-        z[dd] = (dd < nd && kval) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2)) : cx<double>{0.0, 0.0};
+        for (int dd = 0; dd < MAXD; ++dd)
+            z[dd] = (dd < nd && m2 < N2) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2)) : cx<double>{0.0, 0.0};
+    }
+    auto zval = [&](int p, int dd) -> cx<double> {
+        if constexpr (KPW_MAX > 1) return zt[p * MAXD + dd];
+        else return z[dd];
+    };
 
     int wmask = 0;
 #pragma unroll
@@ -1173,13 +1199,20 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         if (DB && ti + 1 < d.nt) build(ti + 1, buf ^ 1);
         const cx<T>* G = reinterpret_cast<const cx<T>*>(smem + L.off_g + buf * L.gbytes);
         const cx<double>* GV = reinterpret_cast<const cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
+        cx<double> jv[3] = {{0, 0}, {0, 0}, {0, 0}};
+        // the k-points of the wave in turn; the state sets rotate through
+        // rhoA[0] and are back in place after KPW_MAX turns
+#pragma unroll 1
+        for (int p = 0; p < KPW_MAX; ++p) {
+        if (p < KPW) {
+        cx<double>* rho = rhoA[0];
         cx<T> Hs[4], rs[4], K[4], acc[4];
         auto hstage = [&](int j) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) Hs[q] = {(T)0, (T)0};
 #pragma unroll
             for (int dd = 0; dd < MAXD; ++dd) {
-                const cx<T> zz = cast<T>(z[dd]);
+                const cx<T> zz = cast<T>(zval(p, dd));
                 const cx<T>* gp = G + (size_t)(j * MAXD + dd) * E;
                 cx<T> g0, g1, g2, g3;
                 TT::ld2(gp, i0, j0, g0, g1);
@@ -1289,7 +1322,6 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             rho[q].im = fmaT(pref, (double)acc[q].im, rho[q].im);
         }
         // current of this step: sum_d tw2[m2 n2_d] sum_ij GV[d][a][ij] conj(rho_ij)
-        cx<double> jv[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
         for (int dd = 0; dd < MAXD; ++dd) {
             if (!((wmask >> dd) & 1)) continue;  // no velocity block at this n2 (uniform)
@@ -1305,8 +1337,19 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 cfmac(s, rho[1], v1);
                 cfmac(s, rho[2], v2);
                 cfmac(s, rho[3], v3);
-                cfma(jv[a], z[dd], s);
+                cfma(jv[a], zval(p, dd), s);
             }
+        }
+        }
+        if constexpr (KPW_MAX > 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const cx<double> r0 = rhoA[0][q];
+#pragma unroll
+                for (int r = 0; r + 1 < KPW_MAX; ++r) rhoA[r][q] = rhoA[r + 1][q];
+                rhoA[KPW_MAX - 1][q] = r0;
+            }
+        }
         }
         double jc[6] = {jv[0].re, jv[0].im, jv[1].re, jv[1].im, jv[2].re, jv[2].im};
 #pragma unroll
@@ -1330,11 +1373,17 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
-        if (kval && ei < nb && ej < nb) {
-            ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k)] = rho[q].re;
-            ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k) + 1] = rho[q].im;
+    for (int p = 0; p < KPW_MAX; ++p) {
+        const int m2 = m2_of(p);
+        const bool kval = p < KPW && m2 < N2;
+        const int k = m1 * N2 + (kval ? m2 : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+            if (kval && ei < nb && ej < nb) {
+                ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k)] = rhoA[p][q].re;
+                ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k) + 1] = rhoA[p][q].im;
+            }
         }
     }
 }
@@ -1764,12 +1813,40 @@ static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
     return m;
 }
 
+static int device_cus() {
+    static const int n = [] {
+        int dev = 0, c = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+        return c > 0 ? c : 1;
+    }();
+    return n;
+}
+
+// k-points per wave of the 16-band row-separable kernel (one k-point per wave
+// otherwise). Its workgroups (one per CU, by LDS) stay resident for the whole
+// time loop, so a solve takes ceil(tiles / CUs) workgroup lifetimes, each one
+// table build (about 1.2 k-point steps) plus KPW k-point steps per step; the
+// KPW of 1, 2, 4 with the least total wins.
+static int sep_kpw(const SolveDesc& d, const LaunchKey& k) {
+    if (k.kind != KIND_SEP || k.a != 16) return 1;
+    const int cus = device_cus();
+    int best = 1;
+    double bcost = 0.0;
+    for (int kp = 1; kp <= 4; kp *= 2) {
+        const long tiles = (long)d.N1 * ((d.N2 + k.b * kp - 1) / (k.b * kp));
+        const double cost = (double)((tiles + cus - 1) / cus) * (1.2 + kp);
+        if (kp == 1 || cost < bcost) { best = kp; bcost = cost; }
+    }
+    return best;
+}
+
 // Tiles (workgroups) of one instance and waves per tile.
 static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block) {
     if (k.kind == KIND_SEP) {
         block = 64 * k.b;
         waves_per_tile = 1;  // one current partial per workgroup and step
-        const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2)));
+        const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2))) * sep_kpw(d, k);
         return d.N1 * ((d.N2 + kpt - 1) / kpt);
     }
     if (k.kind == KIND_ELEM) {
@@ -1859,11 +1936,11 @@ static void ensure_jpart(DeviceSolve* ds, int nwaves, int C) {
     }
 }
 
-static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves) {
+static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves, int kpw) {
     InstPtrs p;
     p.blk = ds->blk; p.steptab = ds->steptab; p.tw1 = ds->tw1; p.tw2 = ds->tw2;
     p.pcoef = ds->pcoef; p.scoef = ds->scoef; p.psteps = ds->psteps; p.rho_in = ds->rho;
-    p.rho_out = ds->rho_tmp; p.jpart = ds->jpart; p.nwaves = nwaves; p.dt = ds->dt;
+    p.rho_out = ds->rho_tmp; p.jpart = ds->jpart; p.nwaves = nwaves; p.kpw = kpw; p.dt = ds->dt;
     return p;
 }
 
@@ -1881,7 +1958,7 @@ static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, 
     nwaves = ntiles * wpt;
     const int C = comps_of(k);
     ensure_jpart(ds, nwaves, C);
-    const InstPtrs ip = ptrs_of(ds, nwaves);
+    const InstPtrs ip = ptrs_of(ds, nwaves, sep_kpw(d, k));
     const size_t lds = lds_of(d, k);
     dispatch(k, [&](auto single, auto) {
         set_lds_limit((const void*)single, lds);
@@ -1986,6 +2063,8 @@ void dev_launch_key(DeviceSolve* ds, int precision, int* out5) {
     const LaunchKey k = classify(ds, precision);
     out5[0] = k.kind; out5[1] = k.a; out5[2] = k.b; out5[3] = k.c; out5[4] = k.fp32;
 }
+
+int dev_kpoints_per_wave(DeviceSolve* ds, int precision) { return sep_kpw(ds->d, classify(ds, precision)); }
 
 void dev_launch(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
@@ -2126,7 +2205,7 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
             DevInst& I = insts[j];
             I.d = ds->d;
             I.pd = ds->pd;
-            I.p = ptrs_of(ds, nwaves);
+            I.p = ptrs_of(ds, nwaves, sep_kpw(ds->d, key));
             I.J = ds->J;
             I.C = comps_of(key);
             const OMap om = omap_of(ds, key);
@@ -2217,7 +2296,7 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         // the k-points of a workgroup
         const LaunchKey k = classify(ds, precision);
         const double P = k.a, P2 = P * P, D = k.c;
-        const double kpt = k.b * (64.0 / ((P / 2) * (P / 2)));
+        const double kpt = k.b * (64.0 / ((P / 2) * (P / 2))) * sep_kpw(d, k);
         const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);
         const double table = 90.0 * d.ns * d.nb * d.nb;
         return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + 3 * D * (P2 / 4) * 40.0 + table / (kpt * used);

@@ -89,6 +89,7 @@ struct InstPtrs {
     const double *blk, *steptab, *tw1, *tw2, *pcoef, *scoef, *psteps, *rho_in;
     double *rho_out, *jpart;
     int nwaves;          // waves (current partials per step) of this instance's launch
+    int kpw;             // k-points per wave of the row-separable kernel (16 bands: 1, 2 or 4)
     double dt;
 };
 

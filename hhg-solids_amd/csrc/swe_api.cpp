@@ -296,6 +296,14 @@ API int SWESim_get_launch(SWESim* s, int* out5) {
     });
     return s ? s->status : -1;
 }
+API int SWESim_get_kpoints_per_wave(SWESim* s) {
+    int kpw = -1;
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_get_kpoints_per_wave: call SWESim_init first");
+        kpw = hhg::dev_kpoints_per_wave(s->dev, s->precision);
+    });
+    return s && s->status == 0 ? kpw : -1;
+}
 API void hhg_set_autotune(int on) { hhg::set_autotune(on); }
 API int hhg_get_autotune() { return hhg::get_autotune(); }
 API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_last_kernel_ms(s->dev) : 0.0; }

@@ -55,6 +55,7 @@ def bind(lib):
         "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
         "SWESim_flops_per_kstep": ([V], Dd), "SWESim_autotune": ([V, I], I),
         "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
+        "SWESim_get_kpoints_per_wave": ([V], I),
         "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
         "hhg_fft2": ([np.ctypeslib.ndpointer(dtype=np.complex128, flags="C"), I, I, I, I], I),
         "hhg_fft2_plan_create": ([I, I, I], V), "hhg_fft2_plan_exec": ([V, V, I, I], Dd),
@@ -252,6 +253,12 @@ class Solve:
         self.lib.SWESim_get_launch(self.ptr, out)
         self._check("get_launch")
         return dict(zip(("kernel", "a", "b", "c", "fp32"), out.tolist()))
+
+    def kpoints_per_wave(self):
+        """k-points per wave of the next run (1, 2 or 4 on the 16-band row-separable kernel)."""
+        k = self.lib.SWESim_get_kpoints_per_wave(self.ptr)
+        self._check("get_kpoints_per_wave")
+        return k
 
     def kernel_ms(self):
         return self.lib.SWESim_last_kernel_ms(self.ptr)

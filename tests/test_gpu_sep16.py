@@ -1,0 +1,81 @@
+"""The 16-band row-separable kernel with several k-points per wave.
+
+When a 16-band grid has more workgroups than the device has CUs, each wave
+carries 2 or 4 k-points of its row through a step under one per-row table
+build (SWESim_get_kpoints_per_wave). The k-points of a wave sit W apart in the
+row, and the last tile of a row may be partly past its end. These runs must
+match the oracle, and batched they must equal their runs alone.
+"""
+import numpy as np
+import pytest
+
+from conftest import make_param, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10   # vs the oracle
+FP32_TOL = 1e-5    # J of FP32 stages vs the FP64 oracle
+SAME_TOL = 1e-13   # batch member vs the same solve alone
+
+# (grid, precision): 16-band grids whose workgroups outnumber the CUs; rows
+# of 6, 20 and 40 k-points end in partial tiles (FP64: 4 waves per workgroup,
+# FP32: 8)
+CASES = [((130, 6), "fp64"), ((60, 20), "fp64"), ((80, 40), "fp64"), ((80, 40), "fp32")]
+
+
+def _model(tmp_path):
+    from hhgamd import tbgen
+    path = str(tmp_path / "syn16_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(16, seed=90))
+    return path
+
+
+def _param(path, nr, prec):
+    return dict(make_param(path, nr=nr, tmax=1.0, tmax_field=0.8, intensity=2e12, pol=(0.6, 0.8, 0.0)),
+                precision=prec)
+
+
+@pytest.mark.parametrize("nr,prec", CASES)
+def test_kpoints_per_wave_vs_oracle(tmp_path, oracle_lib, nr, prec):
+    from hhgamd import lib
+    path = _model(tmp_path)
+    p = _param(path, nr, prec)
+    with lib.Solve(p) as s:
+        kind, kpw = s.kernel(), s.kpoints_per_wave()
+        s.run()
+        J = s.current()[1]
+        rho = s.rho_k()
+    _, Jref = oracle_lib.run(path, p, mode="perk")
+    err = rel_l2(J, Jref)
+    print(f"nr={nr} {prec}: kernel {kind}, {kpw} k-points per wave, rel L2 vs oracle {err:.2e}")
+    assert kind == 6 and kpw in (2, 4)
+    assert err <= (FP64_TOL if prec == "fp64" else FP32_TOL)
+    # every k-point of the grid, partial tiles included, was propagated: rho
+    # stays Hermitian with its trace, and the field moved it off the start
+    with lib.Solve(p, device=False) as s0:
+        rho0 = s0.rho_k()
+    assert np.abs(rho - np.conj(np.swapaxes(rho, 1, 2))).max() < 1e-12
+    tr_tol = 1e-10 if prec == "fp64" else 1e-6  # FP32 stage increments round the trace at ~1e-7
+    assert np.abs(np.trace(rho, axis1=1, axis2=2) - np.trace(rho0, axis1=1, axis2=2)).max() < tr_tol
+    assert np.all(np.abs(rho - rho0).reshape(rho.shape[0], -1).max(axis=1) > 0)
+
+
+def test_kpoints_per_wave_in_a_batch(tmp_path):
+    from hhgamd import lib
+    path = _model(tmp_path)
+    params = [_param(path, nr, prec) for nr, prec in CASES]
+    alone = []
+    for p in params:
+        with lib.Solve(p) as s:
+            s.run()
+            alone.append(s.current()[1])
+    solves = [lib.Solve(p) for p in params]
+    with lib.Batch(solves) as b:
+        b.run()
+    for p, s, Ja in zip(params, solves, alone):
+        Jb = s.current()[1]
+        print(f"nr=({p['nr1']}, {p['nr2']}) {p['precision']}: {s.kpoints_per_wave()} k-points per wave, "
+              f"batch vs alone {rel_l2(Jb, Ja):.1e}")
+        assert s.kpoints_per_wave() in (2, 4)
+        assert rel_l2(Jb, Ja) <= SAME_TOL
+        s.close()
