@@ -912,6 +912,9 @@ struct SepLayout {
 // Tables of NBP <= 8 are double-buffered (built one step ahead); NBP = 16
 // keeps one buffer and builds between two barriers.
 __host__ __device__ constexpr int sep_nbuf(int NBP) { return NBP <= 8 ? 2 : 1; }
+// Most k-points a wave carries through a step under one table build (the
+// launch picks 1 .. this, a power of two)
+__host__ __device__ constexpr int sep_kpw_max(int NBP) { return NBP == 16 ? 4 : NBP == 8 ? 2 : 1; }
 // nstage: support blocks x elements whose H and rbc blocks are staged in LDS
 // (FP32 path, NBP <= 8: ns nb^2), 0 when the table build reads them from HBM.
 __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage) {
@@ -930,7 +933,9 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     L.off_b = o; o = align16(o + nstage * 4 * 8);
     L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
     L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
-    L.off_z = o; o = align16(o + (NBP == 16 ? (size_t)W * 4 * MAXD * 16 : 0));  // row twiddles, 4 k-points per wave
+    // row twiddles of the k-points of each wave when a wave may carry several
+    const int kmax = sep_kpw_max(NBP);
+    L.off_z = o; o = align16(o + (kmax > 1 ? (size_t)W * kmax * GPW * MAXD * 16 : 0));
     L.off_jw = o; o = align16(o + sizeof(double) * 2 * W * 6);
     L.bytes = o;
     return L;
@@ -1033,14 +1038,15 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / LPK, bl = lane - g * LPK;
     const int i0 = 2 * (bl / HB), j0 = 2 * (bl - (bl / HB) * HB);
-    // KPW k-points per wave (NBP = 16: ip.kpw = 1, 2 or 4) share each table
-    // build; k-point p of wave w sits at m2 = base + (p W + w) GPW + g
-    constexpr int KPW_MAX = NBP == 16 ? 4 : 1;
+    // KPW k-points per lane group (ip.kpw, 1 .. sep_kpw_max(NBP)) share each
+    // table build; k-point p of group g of wave w sits at m2 = base + (p W + w) GPW + g
+    constexpr int KPW_MAX = sep_kpw_max(NBP);
     const int KPW = KPW_MAX > 1 ? ip.kpw : 1;
     const int kpt = W * GPW * KPW;
     const int tpr = (N2 + kpt - 1) / kpt;
     const int m1 = tile / tpr;
-    auto m2_of = [&](int p) { return (tile - m1 * tpr) * kpt + (p * W + wave) * GPW + g; };
+    auto m2_at = [&](int p, int gg) { return (tile - m1 * tpr) * kpt + (p * W + wave) * GPW + gg; };
+    auto m2_of = [&](int p) { return m2_at(p, g); };
     cx<T>* bufH = reinterpret_cast<cx<T>*>(smem + L.off_k + (size_t)(wave * GPW + g) * L.kbytes);  // H, then M
     cx<T>* bufX = bufH + E;
     double* jw = reinterpret_cast<double*>(smem + L.off_jw);
@@ -1159,13 +1165,15 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         }
     }
     // row twiddles z_d = tw2[m2 n2_d] (0 for k-points past the row end): in
-    // registers for one k-point per wave, else per wave in LDS (wave-uniform
-    // for NBP = 16, read as broadcasts)
+    // registers for one k-point per lane group, else per wave in LDS, [p][g][d]
+    // (one address per lane group: broadcasts)
     cx<double> z[KPW_MAX > 1 ? 1 : MAXD];
-    cx<double>* zt = reinterpret_cast<cx<double>*>(smem + L.off_z) + (size_t)wave * KPW_MAX * MAXD;
+    cx<double>* zt = reinterpret_cast<cx<double>*>(smem + L.off_z) + (size_t)wave * KPW_MAX * GPW * MAXD;
     if constexpr (KPW_MAX > 1) {
-        if (lane < KPW_MAX * MAXD) {
-            const int p = lane / MAXD, dd = lane - p * MAXD, m2 = m2_of(p);
+        static_assert(KPW_MAX * GPW * MAXD <= 64, "one lane per twiddle of the wave");
+        if (lane < KPW_MAX * GPW * MAXD) {
+            const int p = lane / (GPW * MAXD), r = lane - p * GPW * MAXD, gg = r / MAXD, dd = r - gg * MAXD;
+            const int m2 = m2_at(p, gg);
             zt[lane] = (p < KPW && dd < nd && m2 < N2) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2))
                                                        : cx<double>{0.0, 0.0};
         }
@@ -1176,7 +1184,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             z[dd] = (dd < nd && m2 < N2) ? ld<double>(ip.tw2 + 2 * posmod((long)m2 * d.dn2[dd], N2)) : cx<double>{0.0, 0.0};
     }
     auto zval = [&](int p, int dd) -> cx<double> {
-        if constexpr (KPW_MAX > 1) return zt[p * MAXD + dd];
+        if constexpr (KPW_MAX > 1) return zt[(p * GPW + g) * MAXD + dd];
         else return z[dd];
     };
 
@@ -1206,13 +1214,25 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         for (int p = 0; p < KPW_MAX; ++p) {
         if (p < KPW) {
         cx<double>* rho = rhoA[0];
+        // this k-point's twiddles: registers for NBP <= 8 (read 4x per step),
+        // LDS broadcasts for NBP = 16 (whose FP64 kernel has no registers left)
+        constexpr bool ZREG = KPW_MAX > 1 && NBP != 16;
+        cx<double> zr[ZREG ? MAXD : 1];
+        if constexpr (ZREG) {
+#pragma unroll
+            for (int dd = 0; dd < MAXD; ++dd) zr[dd] = zval(p, dd);
+        }
+        auto zk = [&](int dd) -> cx<double> {
+            if constexpr (ZREG) return zr[dd];
+            else return zval(p, dd);
+        };
         cx<T> Hs[4], rs[4], K[4], acc[4];
         auto hstage = [&](int j) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) Hs[q] = {(T)0, (T)0};
 #pragma unroll
             for (int dd = 0; dd < MAXD; ++dd) {
-                const cx<T> zz = cast<T>(zval(p, dd));
+                const cx<T> zz = cast<T>(zk(dd));
                 const cx<T>* gp = G + (size_t)(j * MAXD + dd) * E;
                 cx<T> g0, g1, g2, g3;
                 TT::ld2(gp, i0, j0, g0, g1);
@@ -1337,7 +1357,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 cfmac(s, rho[1], v1);
                 cfmac(s, rho[2], v2);
                 cfmac(s, rho[3], v3);
-                cfma(jv[a], zval(p, dd), s);
+                cfma(jv[a], zk(dd), s);
             }
         }
         }
@@ -1823,19 +1843,22 @@ static int device_cus() {
     return n;
 }
 
-// k-points per wave of the 16-band row-separable kernel (one k-point per wave
+// k-points per lane group of the row-separable kernel (8 and 16 bands; one
 // otherwise). Its workgroups (one per CU, by LDS) stay resident for the whole
-// time loop, so a solve takes ceil(tiles / CUs) workgroup lifetimes, each one
-// table build (about 1.2 k-point steps) plus KPW k-point steps per step; the
-// KPW of 1, 2, 4 with the least total wins.
+// time loop, so a solve takes ceil(tiles / CUs) workgroup lifetimes, each
+// per step one table build (measured: about 1.2 k-point steps at 16 bands,
+// 0.3 at 8, where it overlaps the step) plus KPW k-point steps; the KPW of
+// 1, 2, 4 with the least total wins.
 static int sep_kpw(const SolveDesc& d, const LaunchKey& k) {
-    if (k.kind != KIND_SEP || k.a != 16) return 1;
+    if (k.kind != KIND_SEP || sep_kpw_max(k.a) == 1) return 1;
     const int cus = device_cus();
+    const int kpt1 = k.b * (64 / ((k.a / 2) * (k.a / 2)));
+    const double build = k.a == 16 ? 1.2 : 0.3;
     int best = 1;
     double bcost = 0.0;
-    for (int kp = 1; kp <= 4; kp *= 2) {
-        const long tiles = (long)d.N1 * ((d.N2 + k.b * kp - 1) / (k.b * kp));
-        const double cost = (double)((tiles + cus - 1) / cus) * (1.2 + kp);
+    for (int kp = 1; kp <= sep_kpw_max(k.a); kp *= 2) {
+        const long tiles = (long)d.N1 * ((d.N2 + kpt1 * kp - 1) / (kpt1 * kp));
+        const double cost = (double)((tiles + cus - 1) / cus) * (build + kp);
         if (kp == 1 || cost < bcost) { best = kp; bcost = cost; }
     }
     return best;

@@ -255,7 +255,7 @@ class Solve:
         return dict(zip(("kernel", "a", "b", "c", "fp32"), out.tolist()))
 
     def kpoints_per_wave(self):
-        """k-points per wave of the next run (1, 2 or 4 on the 16-band row-separable kernel)."""
+        """k-points per lane group of the next run on the row-separable kernel (1, 2 or 4; 1 elsewhere)."""
         k = self.lib.SWESim_get_kpoints_per_wave(self.ptr)
         self._check("get_kpoints_per_wave")
         return k

@@ -86,10 +86,10 @@ int SWESim_autotune(SWESim* swe, int steps);
  * (template parameters; for kernel 6: padded band count, waves per
  * workgroup, table depth). Returns 0 on success. */
 int SWESim_get_launch(SWESim* swe, int* out5);
-/* k-points each wave of the next run carries through a step under one
- * per-row table build: 1, 2 or 4 for the 16-band row-separable kernel (the
- * value with the fewest workgroup lifetimes on this device's CUs), 1 for
- * every other kernel. Returns -1 on error. */
+/* k-points each lane group of the next run carries through a step under one
+ * per-row table build on the row-separable kernel: 1, 2 or 4 at 9-16 bands,
+ * 1 or 2 at 5-8 bands (the value with the fewest workgroup lifetimes on this
+ * device's CUs); 1 for every other kernel. Returns -1 on error. */
 int SWESim_get_kpoints_per_wave(SWESim* swe);
 /* Process-wide: when on, every solve (single, batched or served) is tuned on
  * its first run as by SWESim_autotune(swe, 32). Default off. */

@@ -1,10 +1,11 @@
-"""The 16-band row-separable kernel with several k-points per wave.
+"""The row-separable kernel with several k-points per lane group.
 
-When a 16-band grid has more workgroups than the device has CUs, each wave
-carries 2 or 4 k-points of its row through a step under one per-row table
-build (SWESim_get_kpoints_per_wave). The k-points of a wave sit W apart in the
-row, and the last tile of a row may be partly past its end. These runs must
-match the oracle, and batched they must equal their runs alone.
+When an 8- or 16-band grid has more workgroups than the device has CUs, each
+lane group carries 2 (8 bands) or 2-4 (16 bands) k-points of its row through a
+step under one per-row table build (SWESim_get_kpoints_per_wave). The k-points
+of a group sit W groups apart in the row, and the last tile of a row may be
+partly past its end. These runs must match the oracle, and batched they must
+equal their runs alone.
 """
 import numpy as np
 import pytest
@@ -17,16 +18,17 @@ FP64_TOL = 1e-10   # vs the oracle
 FP32_TOL = 1e-5    # J of FP32 stages vs the FP64 oracle
 SAME_TOL = 1e-13   # batch member vs the same solve alone
 
-# (grid, precision): 16-band grids whose workgroups outnumber the CUs; rows
+# (bands, grid, precision) whose workgroups outnumber the CUs. 16 bands: rows
 # of 6, 20 and 40 k-points end in partial tiles (FP64: 4 waves per workgroup,
-# FP32: 8)
-CASES = [((130, 6), "fp64"), ((60, 20), "fp64"), ((80, 40), "fp64"), ((80, 40), "fp32")]
+# FP32: 8); 8 and 6 (padded to 8) bands: 5 waves of 4 k-points, rows of 36
+CASES = [(16, (130, 6), "fp64"), (16, (60, 20), "fp64"), (16, (80, 40), "fp64"), (16, (80, 40), "fp32"),
+         (8, (160, 36), "fp64"), (8, (160, 36), "fp32"), (6, (160, 36), "fp64")]
 
 
-def _model(tmp_path):
+def _model(tmp_path, nb):
     from hhgamd import tbgen
-    path = str(tmp_path / "syn16_tb.dat")
-    tbgen.write_tb(path, *tbgen.synthetic(16, seed=90))
+    path = str(tmp_path / f"syn{nb}_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=90 + nb))
     return path
 
 
@@ -35,10 +37,10 @@ def _param(path, nr, prec):
                 precision=prec)
 
 
-@pytest.mark.parametrize("nr,prec", CASES)
-def test_kpoints_per_wave_vs_oracle(tmp_path, oracle_lib, nr, prec):
+@pytest.mark.parametrize("nb,nr,prec", CASES)
+def test_kpoints_per_wave_vs_oracle(tmp_path, oracle_lib, nb, nr, prec):
     from hhgamd import lib
-    path = _model(tmp_path)
+    path = _model(tmp_path, nb)
     p = _param(path, nr, prec)
     with lib.Solve(p) as s:
         kind, kpw = s.kernel(), s.kpoints_per_wave()
@@ -47,8 +49,8 @@ def test_kpoints_per_wave_vs_oracle(tmp_path, oracle_lib, nr, prec):
         rho = s.rho_k()
     _, Jref = oracle_lib.run(path, p, mode="perk")
     err = rel_l2(J, Jref)
-    print(f"nr={nr} {prec}: kernel {kind}, {kpw} k-points per wave, rel L2 vs oracle {err:.2e}")
-    assert kind == 6 and kpw in (2, 4)
+    print(f"nb={nb} nr={nr} {prec}: kernel {kind}, {kpw} k-points per lane group, rel L2 vs oracle {err:.2e}")
+    assert kind == 6 and kpw in ((2, 4) if nb > 8 else (2,))
     assert err <= (FP64_TOL if prec == "fp64" else FP32_TOL)
     # every k-point of the grid, partial tiles included, was propagated: rho
     # stays Hermitian with its trace, and the field moved it off the start
@@ -62,8 +64,7 @@ def test_kpoints_per_wave_vs_oracle(tmp_path, oracle_lib, nr, prec):
 
 def test_kpoints_per_wave_in_a_batch(tmp_path):
     from hhgamd import lib
-    path = _model(tmp_path)
-    params = [_param(path, nr, prec) for nr, prec in CASES]
+    params = [_param(_model(tmp_path, nb), nr, prec) for nb, nr, prec in CASES]
     alone = []
     for p in params:
         with lib.Solve(p) as s:
