@@ -1848,9 +1848,12 @@ static int device_cus() {
 // time loop, so a solve takes ceil(tiles / CUs) workgroup lifetimes, each
 // per step one table build (measured: about 1.2 k-point steps at 16 bands,
 // 0.3 at 8, where it overlaps the step) plus KPW k-point steps; the KPW of
-// 1, 2, 4 with the least total wins.
-static int sep_kpw(const SolveDesc& d, const LaunchKey& k) {
-    if (k.kind != KIND_SEP || sep_kpw_max(k.a) == 1) return 1;
+// 1, 2, 4 with the least total wins. In a batch the workgroups of many
+// instances keep every CU busy, so the 8-band gain from fewer lifetimes is
+// absent while the longer ones lengthen the batch's tail (measured 4 % slower
+// on a 64-instance mix): batched solves of up to 8 bands keep one k-point.
+static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch = false) {
+    if (k.kind != KIND_SEP || sep_kpw_max(k.a) == 1 || (batch && k.a <= 8)) return 1;
     const int cus = device_cus();
     const int kpt1 = k.b * (64 / ((k.a / 2) * (k.a / 2)));
     const double build = k.a == 16 ? 1.2 : 0.3;
@@ -1865,11 +1868,11 @@ static int sep_kpw(const SolveDesc& d, const LaunchKey& k) {
 }
 
 // Tiles (workgroups) of one instance and waves per tile.
-static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block) {
+static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block, bool batch = false) {
     if (k.kind == KIND_SEP) {
         block = 64 * k.b;
         waves_per_tile = 1;  // one current partial per workgroup and step
-        const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2))) * sep_kpw(d, k);
+        const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2))) * sep_kpw(d, k, batch);
         return d.N1 * ((d.N2 + kpt - 1) / kpt);
     }
     if (k.kind == KIND_ELEM) {
@@ -2222,13 +2225,13 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
         for (size_t j = 0; j < idx.size(); ++j) {
             DeviceSolve* ds = b->items[idx[j]];
             int wpt = 0;
-            ntiles[j] = tiles_of(ds->d, key, wpt, block);
+            ntiles[j] = tiles_of(ds->d, key, wpt, block, true);
             const int nwaves = ntiles[j] * wpt;
             ensure_jpart(ds, nwaves, comps_of(key));
             DevInst& I = insts[j];
             I.d = ds->d;
             I.pd = ds->pd;
-            I.p = ptrs_of(ds, nwaves, sep_kpw(ds->d, key));
+            I.p = ptrs_of(ds, nwaves, sep_kpw(ds->d, key, true));
             I.J = ds->J;
             I.C = comps_of(key);
             const OMap om = omap_of(ds, key);

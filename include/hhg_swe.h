@@ -89,7 +89,8 @@ int SWESim_get_launch(SWESim* swe, int* out5);
 /* k-points each lane group of the next run carries through a step under one
  * per-row table build on the row-separable kernel: 1, 2 or 4 at 9-16 bands,
  * 1 or 2 at 5-8 bands (the value with the fewest workgroup lifetimes on this
- * device's CUs); 1 for every other kernel. Returns -1 on error. */
+ * device's CUs); 1 for every other kernel. Batched runs (hhg_batch_run) keep
+ * 1 at up to 8 bands. Returns -1 on error. */
 int SWESim_get_kpoints_per_wave(SWESim* swe);
 /* Process-wide: when on, every solve (single, batched or served) is tuned on
  * its first run as by SWESim_autotune(swe, 32). Default off. */
