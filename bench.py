@@ -126,7 +126,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nr", type=int, default=200)
+    ap.add_argument("--nr", type=int, default=None, help="k-grid side (default 120 for --mode precision, else 200)")
+    ap.add_argument("--bands", type=int, default=8, help="precision: bands of the seeded model")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--cpu-steps", type=int, default=40, help="RK4 steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -145,6 +146,8 @@ def main():
     ap.add_argument("--policy", default="least-work", choices=["least-work", "fifo"],
                     help="serve: admission order among prepared requests")
     args = ap.parse_args()
+    if args.nr is None:
+        args.nr = 120 if args.mode == "precision" else 200
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
               "(the k-grid would shard by slabs of k-rows, with one final reduce of J over the GPUs)", file=sys.stderr)
@@ -229,15 +232,16 @@ def main():
 
 
 def bench_precision(args):
-    """Seeded 8-band Wannier model (5x5 R shell), 120x120 k-grid, I = 5e11,
-    readme pulse otherwise: the mixed path (FP32 commutator and RK4 stages,
-    FP64 state, initial state and current) timed, its J(t) compared with the
-    FP64 path of the same solve."""
+    """Seeded Wannier model (--bands, default 8; 5x5 R shell), nr x nr k-grid
+    (default 120), I = 5e11, readme pulse otherwise: the mixed path (FP32
+    commutator and RK4 stages, FP64 state, initial state and current) timed,
+    its J(t) compared with the FP64 path of the same solve."""
     from hhgamd import lib, workloads
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
-    prm = workloads.c3(tmp, precision=args.precision if args.precision != "fp64" else "fp32")
+    prm = workloads.c3(tmp, precision="fp32", nb=args.bands, nr=args.nr)
     s = lib.Solve(prm)
     nk, nt, nb = s.N, s.nt, s.nb
+    kpw = s.kpoints_per_wave()
     F = s.flops_per_kstep()
     for _ in range(args.warmup):
         s.run()
@@ -265,8 +269,8 @@ def bench_precision(args):
     out = {"metric": METRIC, "value": nk * nt * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-           "config": {"workload": f"synthetic-8band-{prm['nr1']}x{prm['nr2']}-nt{nt}-fp32", "nk": nk, "nb": nb,
-                      "nt": nt, "precision": "fp32 stages, fp64 state/current"},
+           "config": {"workload": f"synthetic-{nb}band-{prm['nr1']}x{prm['nr2']}-nt{nt}-fp32", "nk": nk, "nb": nb,
+                      "nt": nt, "precision": "fp32 stages, fp64 state/current", "kpoints_per_lane_group": kpw},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "j_rel_l2_vs_fp64": err, "worst_harmonic_dev_vs_fp64": worst,
            "fp64_value": nk * nt / fp64_s, "fp64_kernel_ms": fp64_kms,

@@ -67,8 +67,10 @@ def c2(workdir, precision="fp64"):
     return readme_pulse(haldane_file(workdir), 200, precision)
 
 
-def c3(workdir, precision="fp32"):
-    p = readme_pulse(synthetic_file(workdir, 8), 120, precision)
+def c3(workdir, precision="fp32", nb=8, nr=120):
+    """Seeded nb-band model (5x5 R shell) on an nr x nr grid, I = 5e11, the
+    README pulse otherwise (the 8-band 120 x 120 case by default)."""
+    p = readme_pulse(synthetic_file(workdir, nb), nr, precision)
     p["intensity"] = 5e11
     return p
 
