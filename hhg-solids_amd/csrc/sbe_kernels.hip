@@ -1112,6 +1112,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) a[o] = {0, 0};
             const int qe = seg[dd + 1];
+#pragma unroll 2  // two supports' block loads in flight (1-4 % at 16 bands, same results)
             for (int qq = seg[dd]; qq < qe; ++qq) {
                 const int sv = ss[qq];
                 const double* bs = ip.blk + 2 * ((size_t)(sv & 0xffff) * NOPS * nbe + el);
