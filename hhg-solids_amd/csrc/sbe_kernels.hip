@@ -1215,7 +1215,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         for (int p = 0; p < KPW_MAX; ++p) {
         if (p < KPW) {
         cx<double>* rho = rhoA[0];
-        // this k-point's twiddles: registers for NBP <= 8 (read 4x per step),
+        // this k-point's twiddles for the three stage sums: registers for NBP <= 8,
         // LDS broadcasts for NBP = 16 (whose FP64 kernel has no registers left)
         constexpr bool ZREG = KPW_MAX > 1 && NBP != 16;
         cx<double> zr[ZREG ? MAXD : 1];
@@ -1342,25 +1342,6 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             rho[q].re = fmaT(pref, (double)acc[q].re, rho[q].re);
             rho[q].im = fmaT(pref, (double)acc[q].im, rho[q].im);
         }
-        // current of this step: sum_d tw2[m2 n2_d] sum_ij GV[d][a][ij] conj(rho_ij)
-#pragma unroll
-        for (int dd = 0; dd < MAXD; ++dd) {
-            if (!((wmask >> dd) & 1)) continue;  // no velocity block at this n2 (uniform)
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (!((d.vmask >> a) & 1)) continue;
-                const cx<double>* vp = GV + (size_t)(dd * 3 + a) * E;
-                cx<double> v0, v1, v2, v3;
-                TD::ld2(vp, i0, j0, v0, v1);
-                TD::ld2(vp, i0 + 1, j0, v2, v3);
-                cx<double> s = {0, 0};
-                cfmac(s, rho[0], v0);
-                cfmac(s, rho[1], v1);
-                cfmac(s, rho[2], v2);
-                cfmac(s, rho[3], v3);
-                cfma(jv[a], zk(dd), s);
-            }
-        }
         }
         if constexpr (KPW_MAX > 1) {
 #pragma unroll
@@ -1371,6 +1352,31 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 rhoA[KPW_MAX - 1][q] = r0;
             }
         }
+        }
+        // current of this step, one pass over GV for all k-points p of the lane
+        // group: sum_p sum_d tw2[m2_p n2_d] sum_ij GV[d][a][ij] conj(rho_p,ij)
+#pragma unroll
+        for (int dd = 0; dd < MAXD; ++dd) {
+            if (!((wmask >> dd) & 1)) continue;  // no velocity block at this n2 (uniform)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (!((d.vmask >> a) & 1)) continue;
+                const cx<double>* vp = GV + (size_t)(dd * 3 + a) * E;
+                cx<double> v0, v1, v2, v3;
+                TD::ld2(vp, i0, j0, v0, v1);
+                TD::ld2(vp, i0 + 1, j0, v2, v3);
+#pragma unroll
+                for (int p = 0; p < KPW_MAX; ++p) {
+                    if (p < KPW) {
+                        cx<double> s = {0, 0};
+                        cfmac(s, rhoA[p][0], v0);
+                        cfmac(s, rhoA[p][1], v1);
+                        cfmac(s, rhoA[p][2], v2);
+                        cfmac(s, rhoA[p][3], v3);
+                        cfma(jv[a], zval(p, dd), s);
+                    }
+                }
+            }
         }
         double jc[6] = {jv[0].re, jv[0].im, jv[1].re, jv[1].im, jv[2].re, jv[2].im};
 #pragma unroll
