@@ -120,6 +120,16 @@ def test_run_without_device_init_is_an_error(native):
             s.run()
 
 
+def test_launch_queries_without_device_init_are_errors(native):
+    """The launch queries describe the device run: before SWESim_init they
+    fail with a message instead of guessing."""
+    with native.Solve(make_param(HMCASE0, nr=8, tmax=1.0), device=False) as s:
+        with pytest.raises(RuntimeError, match="SWESim_init"):
+            s.kpoints_per_wave()
+        with pytest.raises(RuntimeError, match="SWESim_init"):
+            s.launch()
+
+
 def test_bad_precision_rejected(native):
     with pytest.raises(RuntimeError, match="precision"):
         native.Solve(dict(make_param(HMCASE0, nr=8, tmax=1.0), precision="fp16"), device=False)
