@@ -896,6 +896,7 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // m1. Per step it builds, cooperatively and double-buffered in LDS,
 //   G[j][d]  = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} (H_s + E_j . rbc_s)   (stage fields j = 0..2)
 //   GV[d][a] = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} v_a,s                 (axes a = x, y, z)
+// (for NBP = 16 GV holds only the values d with a velocity block, by rank)
 // and every k-point forms its stage Hamiltonian sum_d tw2[m2 n2_d] G[j][d]
 // with MAXD complex MACs per element instead of ns. For Hermitian H and X the
 // commutator is -i(M - M^dagger) with M = H X: one complex product per stage.
@@ -917,7 +918,9 @@ __host__ __device__ constexpr int sep_nbuf(int NBP) { return NBP <= 8 ? 2 : 1; }
 __host__ __device__ constexpr int sep_kpw_max(int NBP) { return NBP == 16 ? 4 : NBP == 8 ? 2 : 1; }
 // nstage: support blocks x elements whose H and rbc blocks are staged in LDS
 // (FP32 path, NBP <= 8: ns nb^2), 0 when the table build reads them from HBM.
-__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage) {
+// nwd: values d with a velocity block; for NBP = 16, where the LDS bounds the
+// waves per workgroup, the velocity tables GV hold only those (by rank)
+__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage, int nwd) {
     SepLayout L;
     const size_t E = (size_t)NBP * NBP;
     const int GPW = 64 / ((NBP / 2) * (NBP / 2));
@@ -925,7 +928,7 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     size_t o = 0;
     L.gbytes = 3 * (size_t)MAXD * E * 2 * tsize;
     L.off_g = o; o = align16(o + nbuf * L.gbytes);
-    L.gvbytes = (size_t)MAXD * 3 * E * 16;
+    L.gvbytes = (size_t)(NBP == 16 ? nwd : MAXD) * 3 * E * 16;
     L.off_gv = o; o = align16(o + nbuf * L.gvbytes);
     L.off_sup = o; o = align16(o + MAX_SUPPORT * 16 + MAX_SUPPORT * 4 + 16 * 4 + (size_t)MAXD * E * 2);
     if (nbuf == 2) { L.off_w = o; o = align16(o + 2 * (MAX_SUPPORT * 16 + 16 * 8)); }  // weights + field rows
@@ -1032,7 +1035,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int W = blockDim.x >> 6;
     // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
     constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8;
-    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0);
+    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd);
     const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1096,6 +1099,12 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     }
     __syncthreads();
 
+    int wmask = 0;  // values d with a velocity block
+#pragma unroll
+    for (int dd = 0; dd < MAXD; ++dd)
+        if (dd < nd && d.wdi[dd] >= 0) wmask |= 1 << dd;
+    // GV slot of value d: its rank among those for NBP = 16, d itself otherwise
+    auto wrank = [&](int dd) { return NBP == 16 ? __popc((unsigned)wmask & ((1u << dd) - 1u)) : dd; };
     auto build = [&](int ti, int buf) {
         const double* st = ip.steptab + (size_t)ti * d.stride;
         double ef[9];
@@ -1146,9 +1155,11 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 }
                 G[(size_t)(j * MAXD + dd) * E + TT::off(ei, ej)] = cast<T>(b);
             }
-            cx<double>* av = a + OP_V;
+            if ((wmask >> dd) & 1) {
+                const int wr = wrank(dd);
 #pragma unroll
-            for (int x = 0; x < 3; ++x) GV[(size_t)(dd * 3 + x) * E + TD::off(ei, ej)] = av[x];
+                for (int x = 0; x < 3; ++x) GV[(size_t)(wr * 3 + x) * E + TD::off(ei, ej)] = a[OP_V + x];
+            }
         }
     };
 
@@ -1189,10 +1200,6 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         else return z[dd];
     };
 
-    int wmask = 0;
-#pragma unroll
-    for (int dd = 0; dd < MAXD; ++dd)
-        if (dd < nd && d.wdi[dd] >= 0) wmask |= 1 << dd;
     const T half = (T)(0.5 * ip.dt);
     const T full = (T)ip.dt;
     const double pref = 0.1666666666666 * ip.dt;
@@ -1361,7 +1368,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 if (!((d.vmask >> a) & 1)) continue;
-                const cx<double>* vp = GV + (size_t)(dd * 3 + a) * E;
+                const cx<double>* vp = GV + (size_t)(wrank(dd) * 3 + a) * E;
                 cx<double> v0, v1, v2, v3;
                 TD::ld2(vp, i0, j0, v0, v1);
                 TD::ld2(vp, i0 + 1, j0, v2, v3);
@@ -1786,9 +1793,9 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w) {
     if (w <= 0) {
         const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
-        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage).bytes > LDS_MAX) --w;
+        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) --w;
     }
-    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage).bytes > LDS_MAX) return k;
+    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) return k;
     k.kind = KIND_SEP;
     k.a = nbp;
     k.b = w;
@@ -1896,7 +1903,8 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
     if (k.kind == KIND_SEP)
-        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0).bytes;
+        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd)
+            .bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = lanes_per_k(d.nb);
     const int kpb = 256 / G;

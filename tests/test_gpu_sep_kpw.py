@@ -19,10 +19,10 @@ FP32_TOL = 1e-5    # J of FP32 stages vs the FP64 oracle
 SAME_TOL = 1e-13   # batch member vs the same solve alone
 
 # (bands, grid, precision) whose workgroups outnumber the CUs. 16 bands: rows
-# of 6, 20 and 40 k-points end in partial tiles (FP64: 4 waves per workgroup,
-# FP32: 8); 12 bands padded to 16 likewise; 8 and 6 (padded to 8) bands: 5
-# waves of 4 k-points, rows of 36
-CASES = [(16, (130, 6), "fp64"), (16, (60, 20), "fp64"), (16, (80, 40), "fp64"), (16, (80, 40), "fp32"),
+# of 6 and 22 (FP64: 5 waves per workgroup) and 40 (FP32: 8 waves) k-points
+# end in partial tiles; 12 bands padded to 16 likewise; 8 and 6 (padded to 8)
+# bands: 5 waves of 4 k-points, rows of 36
+CASES = [(16, (130, 6), "fp64"), (16, (70, 22), "fp64"), (16, (80, 40), "fp64"), (16, (80, 40), "fp32"),
          (12, (130, 6), "fp64"), (12, (80, 40), "fp32"),
          (8, (160, 36), "fp64"), (8, (160, 36), "fp32"), (6, (160, 36), "fp64")]
 
