@@ -86,12 +86,13 @@ def test_field_free_is_stationary(native):
 
 
 # band counts: exact tiles (2, 4, 8, 16) and zero-padded ones (3 -> 4, 5 and 6 -> 8, 12 -> 16), one band;
-# nmax = 3 gives 7 distinct n2 values (the deeper per-row tables of the separable kernel); 20, 24 and 32
-# bands run on the element kernel with a whole workgroup per k-point
+# nmax = 3 gives 7 distinct n2 values (the deeper per-row tables of the separable kernel; at 16 bands the
+# first value has no velocity block, so the velocity tables' slots are the values' ranks shifted by one);
+# 20, 24 and 32 bands run on the element kernel with a whole workgroup per k-point
 @pytest.mark.parametrize("nb,nr,tmax,nmax", [(1, 10, 10.0, 2), (2, 12, 10.0, 2), (3, 8, 6.0, 2), (4, 10, 10.0, 2),
                                              (5, 8, 6.0, 2), (6, 8, 6.0, 2), (8, 12, 10.0, 2), (12, 6, 3.0, 2),
-                                             (16, 6, 3.0, 2), (4, (8, 10), 6.0, 3), (20, 6, 2.0, 2), (24, 4, 1.0, 2),
-                                             (32, (4, 6), 1.0, 2)])
+                                             (16, 6, 3.0, 2), (4, (8, 10), 6.0, 3), (16, 16, 2.0, 3),
+                                             (20, 6, 2.0, 2), (24, 4, 1.0, 2), (32, (4, 6), 1.0, 2)])
 def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, nmax):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
@@ -104,6 +105,9 @@ def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, n
     err = rel_l2(J, J2)
     print(f"nb={nb}: rel L2 vs oracle = {err:.3e}")
     assert err <= FP64_TOL
+    if nb == 16 and nmax == 3:  # 7 n2 values: the row-separable kernel with depth-8 tables
+        with native.Solve(prm) as s:
+            assert s.kernel() == 6 and s.launch()["c"] == 8
     # the trace of every k is conserved (the seeded models may be metallic: occupations differ between k)
     tr0 = np.trace(rho0, axis1=1, axis2=2)
     assert np.abs(tr0 - np.round(tr0.real)).max() < 1e-12

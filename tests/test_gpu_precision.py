@@ -58,3 +58,16 @@ def test_fp32_synthetic(tmp_path, nb, nr):
     tbgen.write_tb(path, *tbgen.synthetic(nb, seed=11 + nb))
     err, worst = check(make_param(path, nr=nr, intensity=5e11, pol=(0.6, 0.8, 0.0)))
     assert err <= J_TOL and worst <= HARMONIC_TOL
+
+
+def test_fp32_synthetic_16_bands_7x7_shell(tmp_path):
+    """16 bands over a 7x7 R shell: 7 n2 values, the first without a velocity
+    block (the velocity tables hold the other values by rank)."""
+    from hhgamd import lib, tbgen
+    path = str(tmp_path / "syn16n3_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(16, seed=31, nmax=3))
+    prm = make_param(path, nr=16, intensity=5e11, pol=(0.6, 0.8, 0.0))
+    with lib.Solve(dict(prm, precision="fp32")) as s:
+        assert s.kernel() == 6 and s.launch()["c"] == 8
+    err, worst = check(prm)
+    assert err <= J_TOL and worst <= HARMONIC_TOL
