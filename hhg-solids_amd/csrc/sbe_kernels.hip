@@ -20,8 +20,10 @@
 //                              complex (window) current.
 //   k_sep<NBP,MAXD,T>          Hermitian 3 <= nb <= 16: row-separable Fourier
 //                              sums built once per k-row and step in LDS,
-//                              2x2 element blocks per lane, M - M^dagger
-//                              commutator (16 bands: M on the matrix cores).
+//                              2x2 element blocks per lane, each lane group
+//                              carrying 1, 2 or 4 k-points of the row per
+//                              build, M - M^dagger commutator (16 bands: M as
+//                              three real products on the matrix cores).
 //   k_thread_per_k<NB,MAXU,T>  one lane per k, everything in registers; the
 //                              blocks and the per-step table are uniform and
 //                              arrive through scalar loads. For nb <= 2.
