@@ -904,8 +904,11 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // commutator is -i(M - M^dagger) with M = H X: one complex product per stage.
 // Each lane owns a 2x2 element block, so every LDS read of the product feeds
 // two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and exchange H,
-// X and M through LDS with wave-level ordering only. One workgroup barrier per
-// step: the tables of step t+1 are built while step t runs (NBP <= 8). The
+// X and M through LDS with wave-level ordering only. Each lane group carries
+// KPW = 1, 2 or 4 k-points of the row (W groups apart) through every step, so
+// one build serves W * GPW * KPW k-points; their states rotate through one
+// register slot and the current reads GV once for all of them. One workgroup
+// barrier per step: the tables of step t+1 are built while step t runs (NBP <= 8). The
 // build reads the per-support weights and field row from LDS (written two
 // steps ahead) and, on the FP32 path, the H and rbc blocks staged in LDS at
 // launch; only the velocity blocks of the window supports come from HBM.
