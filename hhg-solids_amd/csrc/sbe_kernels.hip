@@ -1625,6 +1625,129 @@ __global__ __launch_bounds__(128) void k_init_state(const SolveDesc d, const dou
         }
 }
 
+// The same initial state for 3 <= nb <= 16 with A and Q in LDS: one 16-lane
+// group per k-point, 4 groups per one-wave workgroup (34 KiB of LDS at 16
+// bands, small enough to fit beside other solves' propagation kernels). Lane
+// r of a group owns index r: it forms row r of H_k, applies each rotation's
+// column update to row r of A and Q and its row update to column r of A, and
+// computes the occupation of band r and row r of rho. Each element goes through the
+// operations of k_init_state in the same order (rotation parameters and the
+// convergence sums are evaluated in every lane from the same LDS values, so a
+// group takes one branch), and H_k is formed again for the occupation instead
+// of being kept. Rows are padded to nb + 1 elements so that the 16 lanes of a
+// column access start 4 banks apart.
+constexpr int INIT_GRP = 16, INIT_GPB = 4;
+static size_t init_lds_bytes(int nb) { return (size_t)INIT_GPB * 2 * nb * (nb + 1) * sizeof(double2); }
+
+__global__ __launch_bounds__(64) void k_init_state_lds(const SolveDesc d, const double* __restrict__ blk,
+                                                        const double* __restrict__ tw1, const double* __restrict__ tw2,
+                                                        double* __restrict__ rho_out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nb = d.nb, nb2 = nb * nb, N = d.N, ld = nb + 1;
+    const int grp = threadIdx.x / INIT_GRP, r = threadIdx.x % INIT_GRP;
+    const int k = blockIdx.x * INIT_GPB + grp;
+    const bool kv = k < N;
+    const int kk = kv ? k : N - 1;  // groups past the end repeat the last k-point and store nothing
+    const int m1 = kk / d.N2, m2 = kk - m1 * d.N2;
+    double2* A = reinterpret_cast<double2*>(smem) + (size_t)grp * 2 * nb * ld;
+    double2* Q = A + nb * ld;
+    const bool own = r < nb;
+    // row r of H_k: per element, sum over the supports in order
+    auto form_h_row = [&]() {
+        if (!own) return;
+        for (int j = 0; j < nb; ++j) {
+            double2 acc = make_double2(0.0, 0.0);
+            const int e = r * nb + j;
+            for (int s = 0; s < d.ns; ++s) {
+                if (!(d.opmask[s] & (1u << OP_H))) continue;
+                const int j1 = posmod((long)m1 * d.n1[s], d.N1), j2 = posmod((long)m2 * d.n2[s], d.N2);
+                const double2 w =
+                    zmul(make_double2(tw1[2 * j1], tw1[2 * j1 + 1]), make_double2(tw2[2 * j2], tw2[2 * j2 + 1]));
+                const double* b = blk + 2 * ((size_t)s * NOPS + OP_H) * nb2;
+                acc = zadd(acc, zmul(make_double2(b[2 * e], b[2 * e + 1]), w));
+            }
+            A[r * ld + j] = acc;
+        }
+    };
+    form_h_row();
+    if (own)
+        for (int j = 0; j < nb; ++j) Q[r * ld + j] = make_double2(r == j ? 1.0 : 0.0, 0.0);
+    wave_lds_sync();
+    for (int it = 0; it < 60; ++it) {
+        double off = 0, on = 0;
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) {
+                const double2 x = A[i * ld + j];
+                (i == j ? on : off) += x.x * x.x + x.y * x.y;
+            }
+        if (off <= 1e-32 * on || off == 0.0) break;
+        for (int p = 0; p < nb - 1; ++p)
+            for (int q = p + 1; q < nb; ++q) {
+                const double2 apq = A[p * ld + q];
+                const double g = hypot(apq.x, apq.y);
+                if (g == 0.0) continue;
+                const double2 u = make_double2(apq.x / g, apq.y / g);
+                const double app = A[p * ld + p].x, aqq = A[q * ld + q].x;
+                const double th = 0.5 * (aqq - app) / g;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+                const double2 g00 = make_double2(c, 0.0), g01 = make_double2(sn, 0.0);
+                const double2 g10 = make_double2(-sn * u.x, sn * u.y), g11 = make_double2(c * u.x, -c * u.y);
+                wave_lds_sync();  // every lane has read the parameters' inputs
+                if (own) {  // columns p, q of row r
+                    double2 x = A[r * ld + p], y = A[r * ld + q];
+                    A[r * ld + p] = zadd(zmul(x, g00), zmul(y, g10));
+                    A[r * ld + q] = zadd(zmul(x, g01), zmul(y, g11));
+                    x = Q[r * ld + p];
+                    y = Q[r * ld + q];
+                    Q[r * ld + p] = zadd(zmul(x, g00), zmul(y, g10));
+                    Q[r * ld + q] = zadd(zmul(x, g01), zmul(y, g11));
+                }
+                wave_lds_sync();
+                if (own) {  // rows p, q of column r
+                    const double2 x = A[p * ld + r], y = A[q * ld + r];
+                    A[p * ld + r] = zadd(zmul(zconj(g00), x), zmul(zconj(g10), y));
+                    A[q * ld + r] = zadd(zmul(zconj(g01), x), zmul(zconj(g11), y));
+                }
+                wave_lds_sync();
+                if (r == 0) {
+                    A[p * ld + q] = make_double2(0.0, 0.0);
+                    A[q * ld + p] = make_double2(0.0, 0.0);
+                    A[p * ld + p] = make_double2(A[p * ld + p].x, 0.0);
+                    A[q * ld + q] = make_double2(A[q * ld + q].x, 0.0);
+                }
+                wave_lds_sync();
+            }
+    }
+    // occupation of band r from the diagonal of Q^dagger H Q (H formed again in A)
+    wave_lds_sync();
+    form_h_row();
+    wave_lds_sync();
+    bool occ = false;
+    if (own) {
+        double dn = 0.0;
+        for (int i = 0; i < nb; ++i) {
+            double2 hq = make_double2(0.0, 0.0);
+            for (int j = 0; j < nb; ++j) hq = zadd(hq, zmul(A[i * ld + j], Q[j * ld + r]));
+            dn += zmul(zconj(Q[i * ld + r]), hq).x;
+        }
+        occ = dn < 0.0;
+    }
+    wave_lds_sync();
+    if (own) A[r * ld + r] = make_double2(occ ? 1.0 : 0.0, 0.0);
+    wave_lds_sync();
+    if (own && kv)
+        for (int j = 0; j < nb; ++j) {
+            double2 sum = make_double2(0.0, 0.0);
+            for (int n = 0; n < nb; ++n)
+                if (A[n * ld + n].x != 0.0) sum = zadd(sum, zmul(Q[r * ld + n], zconj(Q[j * ld + n])));
+            rho_out[2 * ((size_t)(r * nb + j) * N + k)] = sum.x;
+            rho_out[2 * ((size_t)(r * nb + j) * N + k) + 1] = sum.y;
+        }
+}
+
+static void set_lds_limit(const void* fn, size_t lds);
+
 struct DeviceSolve {
     SolveDesc d;
     double dt = 0;
@@ -1680,11 +1803,19 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             const SolveDesc& d = hs.d;
             const size_t plane = (size_t)d.nb * d.nb * d.N;
             ds->rho = dalloc(sizeof(double) * 2 * plane, ds->stream);
-            double2* scratch = (double2*)dalloc(sizeof(double2) * 3 * plane, ds->stream);
-            hipLaunchKernelGGL(k_init_state, dim3((d.N + 127) / 128), dim3(128), 0, ds->stream, d, ds->blk, ds->tw1,
-                               ds->tw2, scratch, scratch + plane, scratch + 2 * plane, ds->rho);
-            HIP_CHECK(hipGetLastError());
-            HIP_CHECK(hipFreeAsync(scratch, ds->stream));
+            if (d.nb >= 3 && d.nb <= INIT_GRP) {  // A and Q in LDS, a lane per row
+                const size_t lds = init_lds_bytes(d.nb);
+                set_lds_limit((const void*)k_init_state_lds, lds);
+                hipLaunchKernelGGL(k_init_state_lds, dim3((d.N + INIT_GPB - 1) / INIT_GPB), dim3(INIT_GRP * INIT_GPB),
+                                   lds, ds->stream, d, ds->blk, ds->tw1, ds->tw2, ds->rho);
+                HIP_CHECK(hipGetLastError());
+            } else {
+                double2* scratch = (double2*)dalloc(sizeof(double2) * 3 * plane, ds->stream);
+                hipLaunchKernelGGL(k_init_state, dim3((d.N + 127) / 128), dim3(128), 0, ds->stream, d, ds->blk,
+                                   ds->tw1, ds->tw2, scratch, scratch + plane, scratch + 2 * plane, ds->rho);
+                HIP_CHECK(hipGetLastError());
+                HIP_CHECK(hipFreeAsync(scratch, ds->stream));
+            }
         }
         ds->pd = hs.pd;
         std::vector<double> sc;  // alive until the stream synchronisation below
