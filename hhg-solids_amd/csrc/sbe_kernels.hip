@@ -33,6 +33,9 @@
 //                              stage rho are staged through LDS for the
 //                              commutator. Any model, nb <= 32.
 //   k_reduce_j                 deterministic sum of the per-wave partials.
+//   k_init_state_lds           initial rho (H_k, Jacobi, T = 0 projector) for
+//                              3 <= nb <= 16, a 16-lane group per k in LDS;
+//                              k_init_state (thread per k) for the others.
 // T is the stage arithmetic (double for the plain path, float for the mixed
 // path); the state rho and the current are always FP64.
 #include <hip/hip_runtime.h>
