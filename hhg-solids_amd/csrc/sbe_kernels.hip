@@ -34,8 +34,8 @@
 //                              commutator. Any model, nb <= 32.
 //   k_reduce_j                 deterministic sum of the per-wave partials.
 //   k_init_state_lds           initial rho (H_k, Jacobi, T = 0 projector) for
-//                              3 <= nb <= 16, a 16-lane group per k in LDS;
-//                              k_init_state (thread per k) for the others.
+//                              3 <= nb <= 32, a 16- or 32-lane group per k in
+//                              LDS; k_init_state (thread per k) for nb <= 2.
 // T is the stage arithmetic (double for the plain path, float for the mixed
 // path); the state rho and the current are always FP64.
 #include <hip/hip_runtime.h>
@@ -1628,9 +1628,10 @@ __global__ __launch_bounds__(128) void k_init_state(const SolveDesc d, const dou
         }
 }
 
-// The same initial state for 3 <= nb <= 16 with A and Q in LDS: one 16-lane
-// group per k-point, 4 groups per one-wave workgroup (34 KiB of LDS at 16
-// bands, small enough to fit beside other solves' propagation kernels). Lane
+// The same initial state for 3 <= nb <= 32 with A and Q in LDS: one group of
+// 16 lanes (nb <= 16) or 32 lanes per k-point, 4 or 2 groups per one-wave
+// workgroup (34 KiB of LDS at 16 bands, small enough to fit beside other
+// solves' propagation kernels; 66 KiB at 32). Lane
 // r of a group owns index r: it forms row r of H_k, applies each rotation's
 // column update to row r of A and Q and its row update to column r of A, and
 // computes the occupation of band r and row r of rho. Each element goes through the
@@ -1639,16 +1640,17 @@ __global__ __launch_bounds__(128) void k_init_state(const SolveDesc d, const dou
 // group takes one branch), and H_k is formed again for the occupation instead
 // of being kept. Rows are padded to nb + 1 elements so that the 16 lanes of a
 // column access start 4 banks apart.
-constexpr int INIT_GRP = 16, INIT_GPB = 4;
-static size_t init_lds_bytes(int nb) { return (size_t)INIT_GPB * 2 * nb * (nb + 1) * sizeof(double2); }
+constexpr int INIT_MAXNB = 32;
+__host__ __device__ inline int init_group(int nb) { return nb <= 16 ? 16 : 32; }
+static size_t init_lds_bytes(int nb) { return (size_t)(64 / init_group(nb)) * 2 * nb * (nb + 1) * sizeof(double2); }
 
 __global__ __launch_bounds__(64) void k_init_state_lds(const SolveDesc d, const double* __restrict__ blk,
                                                         const double* __restrict__ tw1, const double* __restrict__ tw2,
                                                         double* __restrict__ rho_out) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nb = d.nb, nb2 = nb * nb, N = d.N, ld = nb + 1;
-    const int grp = threadIdx.x / INIT_GRP, r = threadIdx.x % INIT_GRP;
-    const int k = blockIdx.x * INIT_GPB + grp;
+    const int gs = init_group(nb), grp = threadIdx.x / gs, r = threadIdx.x % gs;
+    const int k = blockIdx.x * (64 / gs) + grp;
     const bool kv = k < N;
     const int kk = kv ? k : N - 1;  // groups past the end repeat the last k-point and store nothing
     const int m1 = kk / d.N2, m2 = kk - m1 * d.N2;
@@ -1806,11 +1808,12 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             const SolveDesc& d = hs.d;
             const size_t plane = (size_t)d.nb * d.nb * d.N;
             ds->rho = dalloc(sizeof(double) * 2 * plane, ds->stream);
-            if (d.nb >= 3 && d.nb <= INIT_GRP) {  // A and Q in LDS, a lane per row
+            if (d.nb >= 3 && d.nb <= INIT_MAXNB) {  // A and Q in LDS, a lane per row
                 const size_t lds = init_lds_bytes(d.nb);
+                const int gpb = 64 / init_group(d.nb);
                 set_lds_limit((const void*)k_init_state_lds, lds);
-                hipLaunchKernelGGL(k_init_state_lds, dim3((d.N + INIT_GPB - 1) / INIT_GPB), dim3(INIT_GRP * INIT_GPB),
-                                   lds, ds->stream, d, ds->blk, ds->tw1, ds->tw2, ds->rho);
+                hipLaunchKernelGGL(k_init_state_lds, dim3((d.N + gpb - 1) / gpb), dim3(64), lds, ds->stream, d,
+                                   ds->blk, ds->tw1, ds->tw2, ds->rho);
                 HIP_CHECK(hipGetLastError());
             } else {
                 double2* scratch = (double2*)dalloc(sizeof(double2) * 3 * plane, ds->stream);
