@@ -456,9 +456,9 @@ def bench_serve(args):
     with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8, policy=args.policy) as srv:
         warm = workloads.c5(tmp, 2, rate=100.0, seed=99, precision=args.precision)
         serve.run_stream(srv, [(0.0, p) for _, p in warm])
-        for rate in rates:
+        for ri, rate in enumerate(rates):
             arrivals = workloads.c5(tmp, args.requests, rate=rate, seed=50, precision=args.precision)
-            _, summ = serve.run_stream(srv, arrivals)
+            recs, summ = serve.run_stream(srv, arrivals, keep_results=ri == len(rates) - 1)
             summ["rate_per_s"] = rate
             sweep.append(summ)
             print(json.dumps({"serve_rate": rate, **summ}), file=sys.stderr, flush=True)
@@ -466,12 +466,16 @@ def bench_serve(args):
     from hhgamd import lib
     top = len(rates) - 1
     plain_s, plain_ksteps = 0.0, 0
-    for _, p in workloads.c5(tmp, args.requests, rate=rates[top], seed=50, precision="fp64"):
+    errs = []  # each served result of the top rate against the same request on the plain FP64 path
+    for rec, (_, p) in zip(recs, workloads.c5(tmp, args.requests, rate=rates[top], seed=50, precision="fp64")):
         with lib.Solve(p) as s:
             a = time.perf_counter()
             s.run()
             plain_s += time.perf_counter() - a
             plain_ksteps += s.N * s.nt
+            if "J" in rec:
+                t64, J64 = s.current()
+                errs.append(j_errors(t64, rec["J"], J64, p["lambda"]))
     best = max(sweep, key=lambda x: x["ksteps_per_s"])
     out = {"metric": METRIC, "value": best["ksteps_per_s"], "unit": UNIT, "n_gpus": 1, "steps": args.requests,
            "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -481,6 +485,9 @@ def bench_serve(args):
            "latency_ms": {"p50": best["p50_latency_s"] * 1e3, "p99": best["p99_latency_s"] * 1e3},
            "plain_fp64_one_by_one_value": plain_ksteps / plain_s,
            "vs_plain_fp64_at_top_rate": sweep[top]["ksteps_per_s"] / (plain_ksteps / plain_s),
+           "served_vs_fp64": {"results": len(errs),
+                              "max_j_rel_l2": max((e[0] for e in errs), default=None),
+                              "worst_harmonic_dev": max((e[1] for e in errs), default=None)},
            "sweep": sweep}
     print(json.dumps(out), flush=True)
 

@@ -116,9 +116,11 @@ class Server:
         self.close()
 
 
-def run_stream(server, arrivals):
+def run_stream(server, arrivals, keep_results=False):
     """Submit ``[(t_arrival_s, param)]`` at their arrival times (relative to
-    now), wait for all, and return per-request records and a summary."""
+    now), wait for all, and return per-request records and a summary. With
+    ``keep_results`` each completed record also holds its ``t`` and ``J``
+    (fetched after the wait; the latencies come from the server's clock)."""
     ids = []
     t0 = time.perf_counter()
     for ta, p in arrivals:
@@ -134,6 +136,8 @@ def run_stream(server, arrivals):
                      "queue_s": tm["admitted"] - tm["submit"] if tm["admitted"] > 0 else None,
                      "device_ms": tm["device_ms"], "ksteps": tm["ksteps"], "nb_grid": (p["nr1"], p["nr2"]),
                      "t_done": tm["done"], "t_submit": tm["submit"]})
+        if keep_results and st == "done":
+            recs[-1]["t"], recs[-1]["J"] = server.result(rid)
     ok = [r for r in recs if r["state"] == "done"]
     lat = np.array([r["latency_s"] for r in ok]) if ok else np.zeros(1)
     span = (max(r["t_done"] for r in ok) - min(r["t_submit"] for r in recs)) if ok else 0.0
