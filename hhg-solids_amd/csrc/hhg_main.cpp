@@ -86,8 +86,8 @@ int main(int argc, char** argv) {
                 if (!f) { std::fprintf(stderr, "hhg_swe: cannot write %s\n", out.c_str()); rc = 1; }
                 else {
                     for (int i = 0; i < nt; ++i)
-                        std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", t[i], jx[i].re, jx[i].im,
-                                     jy[i].re, jy[i].im, jz[i].re, jz[i].im);
+                        std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", t[i], jx[i].real(), jx[i].imag(),
+                                     jy[i].real(), jy[i].imag(), jz[i].real(), jz[i].imag());
                     std::fclose(f);
                 }
             }

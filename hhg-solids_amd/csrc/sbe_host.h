@@ -109,16 +109,20 @@ void dev_launch_key(DeviceSolve* ds, int precision, int* out5);
 int dev_kpoints_per_wave(DeviceSolve* ds, int precision);
 // Flops per k-point and RK4 step of the formulation the next run launches.
 double dev_flops_per_kstep(DeviceSolve* ds, int precision);
+// Arithmetic per k-point and RK4 step fixed by the model (Hermitian folding,
+// live field and current axes), independent of the kernel.
+double dev_min_flops_per_kstep(DeviceSolve* ds);
 void dev_destroy(DeviceSolve* ds);
 
-// Batched propagation: all added solves advance nt steps in one launch per
-// kernel class (different classes overlap on separate streams).
+// Batched propagation: the given solves advance nt steps in one launch per
+// kernel class (different classes overlap on separate streams). A Batch holds
+// streams, events and launch tables across runs, not the solves: the caller
+// names the members (distinct solves) and their precisions at every run.
 struct Batch;
 Batch* batch_create();
 void batch_destroy(Batch* b);
-void batch_add(Batch* b, DeviceSolve* ds, int precision);
-int batch_size(Batch* b);
-// J_out[i]: [nt_i][6] for the i-th added solve. Returns device time (ms).
-double batch_run(Batch* b, const std::vector<double*>& J_out);
+// J_out[i]: [nt_i][6] for items[i]. Returns device time (ms).
+double batch_run(Batch* b, const std::vector<DeviceSolve*>& items, const std::vector<int>& prec,
+                 const std::vector<double*>& J_out);
 
 }  // namespace hhg

@@ -536,6 +536,228 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
 }
 
 // --------------------------------------------------------------------------
+// nb = 2 with Hermitian blocks and the same trace at every k (a filled band
+// at T = 0): the Bloch-vector form of the register-pair kernel. With
+// rho = r0 I + s.sigma and H = h0 I + h.sigma, -i [H, rho] = (2 h x s).sigma,
+// so r0 is constant, h0 drops out of the dynamics and a stage is one cross
+// product g x s with g = 2 h = (2 Re b, -2 Im b, a - d) for H = [a b; b* d]
+// (rho = [p c; c* q]: s = (Re c, -Im c, (p - q)/2), r0 = (p + q)/2). The
+// current of a Hermitian window, J = m0 p + m1 q + m2 Re c + m3 Im c, splits
+// into r0 (m0 + m1) -- summed over the grid this is the constant
+// N r0 (m0 + m1) of the origin, since the k-sum of e^{-i k.R} vanishes for
+// R != 0, added by the reduction -- and (m0 - m1) s_z + m2 s_x - m3 s_y.
+// Per k-point and step: 3 outputs per Fourier sum instead of 4, 6 VALU
+// operations per stage derivative instead of ~9, 3 state components.
+template <int NREP, int NX, int NV, typename T>
+__device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
+    const cdptr coef = as_const(ip.pcoef);
+    const double* __restrict__ tw1 = ip.tw1;
+    const double* __restrict__ tw2 = ip.tw2;
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
+    const int k = bid * 64 + threadIdx.x;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = bid;
+    const int lane = threadIdx.x & 63;
+    constexpr int TCH = 16;
+    using JS = JStage<3, TCH>;
+    __shared__ double jl[64 * JS::ROW];
+    constexpr int NXA = NX > 0 ? NX : 1;
+
+    // Per pair and operator the 4 outputs (a, d, Re b, Im b) have coefficients
+    // (c0, c1) of (Re w, Im w); with the twiddle w0 = e^{-i k.R} folded in
+    // (w = w0 phi) and mapped to the Bloch components:
+    //   g = (2 Re b, -2 Im b, a - d),   current u = (m2, -m3, m0 - m1).
+    const double lz = 0.0 * (double)lane;
+    T hc[NREP][6], xc[NREP][NXA][6];
+    double vc[NREP][NV][6];
+    auto fold = [](const double c0, const double c1, const cx<double> w0, double& a, double& b) {
+        a = c0 * w0.re + c1 * w0.im;
+        b = c1 * w0.re - c0 * w0.im;
+    };
+    // 4 outputs x 2 folded coefficients -> 3 Bloch components x 2
+    auto bloch = [&](const cdptr c, const cx<double> w0, const double sx, const double sy, double* out) {
+        double f[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fold(c[2 * j], c[2 * j + 1], w0, f[j][0], f[j][1]);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            out[u] = sx * f[2][u];
+            out[2 + u] = sy * f[3][u];
+            out[4 + u] = f[0][u] - f[1][u];
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < NREP; ++r) {
+        const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
+        const cx<double> w0 = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
+        const cdptr cr = coef + PAIR_HEAD + PAIR_REP * r;
+        double o[6];
+        bloch(cr, w0, 2.0, -2.0, o);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) hc[r][u] = (T)o[u];
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+            bloch(cr + 8 + 8 * x, w0, 2.0, -2.0, o);
+#pragma unroll
+            for (int u = 0; u < 6; ++u) xc[r][x][u] = (T)o[u];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) bloch(cr + 32 + 8 * v, w0, 1.0, -1.0, vc[r][v]);
+    }
+    // origin terms (coefficients of the constant phase 1)
+    T g0[3], x0[NXA][3];
+    double u0[NV][3];
+    g0[0] = (T)(2.0 * coef[2] + lz); g0[1] = (T)(-2.0 * coef[3] + lz); g0[2] = (T)(coef[0] - coef[1] + lz);
+#pragma unroll
+    for (int x = 0; x < NX; ++x) {
+        const cdptr c = coef + 4 + 4 * x;
+        x0[x][0] = (T)(2.0 * c[2] + lz); x0[x][1] = (T)(-2.0 * c[3] + lz); x0[x][2] = (T)(c[0] - c[1] + lz);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const cdptr c = coef + 16 + 4 * v;
+        u0[v][0] = c[2] + lz; u0[v][1] = -c[3] + lz; u0[v][2] = c[0] - c[1] + lz;
+    }
+
+    double r0 = 0, sx = 0, sy = 0, sz = 0;
+    if (active) {
+        const double p = rho_in[2 * (size_t)k];
+        const cx<double> c = ld<double>(rho_in + 1 * 2 * (size_t)d.N + 2 * k);
+        const double q = rho_in[3 * 2 * (size_t)d.N + 2 * k];
+        r0 = 0.5 * (p + q);
+        sz = 0.5 * (p - q);
+        sx = c.re;
+        sy = -c.im;
+    }
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    // Step state: the row of the step table (pair phases, field) and the
+    // field-free Fourier sums g and X; two buffers alternate as in k_pair2.
+    struct Buf {
+        double ph[NREP][2], ef[3][NXA];
+        T gs[3], xs[NXA][3];
+    };
+    auto sums = [&](Buf& b) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b.gs[j] = g0[j];
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) b.gs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, b.gs[j]));
+        }
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) b.xs[x][j] = x0[x][j];
+#pragma unroll
+            for (int r = 0; r < NREP; ++r) {
+                const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) b.xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, b.xs[x][j]));
+            }
+        }
+    };
+    const int zl = lane * (int)(d.nt < 0);
+    auto load_row = [&](int ti, Buf& b) {
+        const double* row = ip.psteps + (size_t)ti * pd.stride + zl;
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) { b.ph[r][0] = row[PAIR_STEP_HEAD + 2 * r]; b.ph[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) b.ef[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
+    };
+
+    auto step = [&](int ti, const Buf& cur, Buf& nxt) {
+        if (ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+        load_row(ti + 1 < d.nt ? ti + 1 : ti, nxt);
+        T gx, gy, gz;
+        auto stage_g = [&](int j) {
+            gx = cur.gs[0]; gy = cur.gs[1]; gz = cur.gs[2];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const T f = (T)cur.ef[j][x];
+                gx = fmaT(f, cur.xs[x][0], gx); gy = fmaT(f, cur.xs[x][1], gy); gz = fmaT(f, cur.xs[x][2], gz);
+            }
+        };
+        T kx, ky, kz;
+        auto deriv = [&](T ax, T ay, T az) {  // k = g x s
+            kx = fmaT(gy, az, -(gz * ay));
+            ky = fmaT(gz, ax, -(gx * az));
+            kz = fmaT(gx, ay, -(gy * ax));
+        };
+        const T Sx = (T)sx, Sy = (T)sy, Sz = (T)sz;
+        stage_g(0);
+        deriv(Sx, Sy, Sz);
+        T ax = kx, ay = ky, az = kz;
+        stage_g(1);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        stage_g(2);
+        deriv(fmaT(full, kx, Sx), fmaT(full, ky, Sy), fmaT(full, kz, Sz));
+        ax += kx; ay += ky; az += kz;
+        sx = fmaT(pref, (double)ax, sx);
+        sy = fmaT(pref, (double)ay, sy);
+        sz = fmaT(pref, (double)az, sz);
+        // current of the new state with the phases of step ti (FP64), without
+        // the r0 term; inactive lanes hold s = 0 and contribute exactly 0
+        double jc[3] = {0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double u[3] = {u0[v][0], u0[v][1], u0[v][2]};
+#pragma unroll
+            for (int r = 0; r < NREP; ++r)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) u[j] = fmaT(vc[r][v][2 * j + 1], cur.ph[r][1], fmaT(vc[r][v][2 * j], cur.ph[r][0], u[j]));
+            jc[v] = fmaT(u[2], sz, fmaT(u[1], sy, u[0] * sx));
+        }
+        JS::put(jl, lane, ti % TCH, jc);
+        sums(nxt);
+    };
+
+    Buf B0, B1;
+    load_row(0, B0);
+    sums(B0);
+    constexpr int CREGS = NREP * 6 * (1 + NX) * (int)(sizeof(T) / 4) + NREP * NV * 12;
+    if constexpr (CREGS <= 160) {
+        int ti = 0;
+        for (; ti + 2 <= d.nt; ti += 2) {
+            step(ti, B0, B1);
+            step(ti + 1, B1, B0);
+        }
+        if (ti < d.nt) step(ti, B0, B1);
+    } else {
+        for (int ti = 0; ti < d.nt; ++ti) {
+            step(ti, B0, B1);
+            B0 = B1;
+        }
+    }
+    if (d.nt > 0) {
+        const int t0 = ((d.nt - 1) / TCH) * TCH;
+        JS::flush(jl, lane, t0, d.nt - t0, jpart, nwaves, wave);
+    }
+    if (active) {
+        double* o = rho_out + 2 * (size_t)k;
+        const size_t st = 2 * (size_t)d.N;
+        o[0] = r0 + sz; o[1] = 0.0;
+        o[st] = sx; o[st + 1] = -sy;
+        o[2 * st] = sx; o[2 * st + 1] = sy;
+        o[3 * st] = r0 - sz; o[3 * st + 1] = 0.0;
+    }
+}
+
+// --------------------------------------------------------------------------
 // nb = 2, Hermitian H and rbc, current possibly complex: pairs streamed.
 // The phases of up to NREP pairs stay in VGPRs; the coefficients (PS_REP per
 // pair) arrive through scalar loads every step -- the coefficient pointer is
@@ -1454,6 +1676,15 @@ __global__ __launch_bounds__(64) void k_pair2_batch(const DevInst* __restrict__ 
     pair2_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
 }
 template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_bloch2(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
+    bloch2_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
+}
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_bloch2_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    bloch2_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
+}
+template <int NREP, int NX, int NV, typename T>
 __global__ __launch_bounds__(64) void k_pair2s(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
     pair2s_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
 }
@@ -1487,6 +1718,7 @@ __global__ __launch_bounds__(256) void k_elem_per_lane_batch(const DevInst* __re
 // output slots are re/im of x, y, z (0..5), unused slots are zero.
 struct OMap {
     int o[6];
+    double cst[6];  // added to component c after the 1/N normalisation
 };
 __global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpart, int nwaves, int C, OMap om,
                                                  double N, double* __restrict__ J) {
@@ -1503,7 +1735,7 @@ __global__ __launch_bounds__(64) void k_reduce_j(const double* __restrict__ jpar
         double out[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            if (c < C && om.o[c] >= 0) out[om.o[c]] = acc[c] / N;
+            if (c < C && om.o[c] >= 0) out[om.o[c]] = acc[c] / N + om.cst[c];
 #pragma unroll
         for (int c = 0; c < 6; ++c) J[(size_t)ti * 6 + c] = out[c];
     }
@@ -1526,7 +1758,7 @@ __global__ __launch_bounds__(64) void k_reduce_j_batch(const DevInst* __restrict
         double out[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            if (c < C && I.omap[c] >= 0) out[I.omap[c]] = acc[c] / N;
+            if (c < C && I.omap[c] >= 0) out[I.omap[c]] = acc[c] / N + I.jcst[c];
 #pragma unroll
         for (int c = 0; c < 6; ++c) I.J[(size_t)t.y * 6 + c] = out[c];
     }
@@ -1752,6 +1984,9 @@ __global__ __launch_bounds__(64) void k_init_state_lds(const SolveDesc d, const 
 }
 
 static void set_lds_limit(const void* fn, size_t lds);
+// The device initial state of 3..INIT_MAXNB bands keeps its matrices in LDS;
+// 1-2 bands use the thread-per-k kernel over 3 element-major scratch planes.
+static bool init_needs_scratch(int nb) { return !(nb >= 3 && nb <= INIT_MAXNB); }
 
 struct DeviceSolve {
     SolveDesc d;
@@ -1768,6 +2003,10 @@ struct DeviceSolve {
     bool use_herm = true;   // paired Hermitian kernel when the blocks allow it
     bool use_sep = true;    // row-separable kernel for Hermitian 3 <= nb <= 16
     bool tuned[2] = {false, false};  // autotuned launch key per precision (FP64, FP32)
+    // 2 bands: every k-point has the same trace 2 r0 (the Bloch-vector kernel
+    // applies); jr0[v] = r0 (m0 + m1) of the origin window block of v slot v
+    bool r0_uniform = false;
+    double jr0[3] = {0, 0, 0};
     int tkey[2][4] = {};
     float last_kernel_ms = 0.f;
 };
@@ -1808,7 +2047,7 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             const SolveDesc& d = hs.d;
             const size_t plane = (size_t)d.nb * d.nb * d.N;
             ds->rho = dalloc(sizeof(double) * 2 * plane, ds->stream);
-            if (d.nb >= 3 && d.nb <= INIT_MAXNB) {  // A and Q in LDS, a lane per row
+            if (!init_needs_scratch(d.nb)) {  // A and Q in LDS, a lane per row
                 const size_t lds = init_lds_bytes(d.nb);
                 const int gpb = 64 / init_group(d.nb);
                 set_lds_limit((const void*)k_init_state_lds, lds);
@@ -1834,6 +2073,30 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             for (int i = 0; i < SC_N; ++i) f[i] = (float)hs.scoef[i];
             ds->scoef = upload(sc, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
+        }
+        if (hs.d.nb == 2 && hs.pd.nrep > 0 && hs.pd.vpaired && !hs.pcoef.empty()) {
+            // r0_k = (rho00 + rho11) / 2 of the initial state
+            const size_t N = hs.d.N;
+            std::vector<double> pq(4 * N);
+            if (!hs.rho0.empty()) {
+                std::copy(hs.rho0.begin(), hs.rho0.begin() + 2 * N, pq.begin());
+                std::copy(hs.rho0.begin() + 6 * N, hs.rho0.begin() + 8 * N, pq.begin() + 2 * N);
+            } else {
+                HIP_CHECK(hipMemcpyAsync(pq.data(), ds->rho, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, ds->stream));
+                HIP_CHECK(hipMemcpyAsync(pq.data() + 2 * N, ds->rho + 6 * N, sizeof(double) * 2 * N,
+                                         hipMemcpyDeviceToHost, ds->stream));
+                HIP_CHECK(hipStreamSynchronize(ds->stream));
+            }
+            double lo = 1e300, hi = -1e300, sum = 0.0;
+            for (size_t k = 0; k < N; ++k) {
+                const double r0 = 0.5 * (pq[2 * k] + pq[2 * N + 2 * k]);
+                lo = std::min(lo, r0);
+                hi = std::max(hi, r0);
+                sum += r0;
+            }
+            ds->r0_uniform = hi - lo <= 1e-12;
+            const double r0 = sum / (double)N;
+            for (int v = 0; v < hs.pd.nv; ++v) ds->jr0[v] = r0 * (hs.pcoef[16 + 4 * v] + hs.pcoef[16 + 4 * v + 1]);
         }
         ds->rho_tmp = dalloc(sizeof(double) * 2 * (size_t)hs.d.nb * hs.d.nb * hs.d.N, ds->stream);
         ds->J = dalloc(sizeof(double) * 6 * (size_t)hs.d.nt, ds->stream);
@@ -1867,16 +2130,16 @@ size_t dev_bytes(const HostSolve& hs) {
     const int G = lanes_per_k(d.nb);
     const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
     // rho and its scratch twin (2 x 2 nb^2 N doubles); a device-side initial
-    // state holds 3 more nb^2 N complex planes while it runs
+    // state of 1-2 bands holds 3 more nb^2 N complex planes while it runs
     const size_t rho = 2 * (size_t)E * d.N;
-    const size_t init = hs.rho0.empty() ? 6 * (size_t)E * d.N : 0;
+    const size_t init = (hs.rho0.empty() && init_needs_scratch(d.nb)) ? 6 * (size_t)E * d.N : 0;
     return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * rho + init +
                              hs.pcoef.size() + hs.scoef.size() * 3 / 2 + 1 + hs.psteps.size() + 6 * (size_t)d.nt +
                              6 * (size_t)d.nt * waves);
 }
 
 // Which kernel and template instance a solve runs with.
-enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6 };
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8 };
 constexpr size_t LDS_MAX = 160 * 1024;
 struct LaunchKey {
     int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
@@ -1898,6 +2161,12 @@ static LaunchKey key_pair(const DeviceSolve* ds, int fp32) {  // coefficients in
     k.a = pd.nrep <= 3 ? 3 : 4;
     if (pd.nx == 0) { k.b = 0; k.c = pd.nv <= 2 ? 2 : 3; }
     else { k.b = 2; k.c = 2; }
+    return k;
+}
+static LaunchKey key_bloch(const DeviceSolve* ds, int fp32) {  // Bloch-vector form of key_pair
+    LaunchKey k = key_pair(ds, fp32);
+    if (k.kind && ds->r0_uniform) k.kind = KIND_BLOCH;
+    else k.kind = 0;
     return k;
 }
 static LaunchKey key_pairs(const DeviceSolve* ds, int fp32) {  // streamed coefficients
@@ -1950,6 +2219,7 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
     const int fp32 = precision != 0;
     LaunchKey k;
     if (!reg_pairs_ok(ds->pd) && (k = key_pairs(ds, fp32)).kind) return k;
+    if ((k = key_bloch(ds, fp32)).kind) return k;
     if ((k = key_pair(ds, fp32)).kind) return k;
     if ((k = key_thread(ds, fp32)).kind) return k;
     if (d.nb > 32) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 32)");
@@ -1974,16 +2244,22 @@ static LaunchKey classify(const DeviceSolve* ds, int precision) {
 }
 
 static int comps_of(const LaunchKey& k) {
-    if (k.kind == KIND_PAIR) return 3;
+    if (k.kind == KIND_PAIR || k.kind == KIND_BLOCH) return 3;
     if (k.kind == KIND_PAIRS) return 2 * k.c;
     return 6;
 }
 
 static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
     OMap m;
-    for (int c = 0; c < 6; ++c) m.o[c] = (k.kind == KIND_PAIR || k.kind == KIND_PAIRS) ? -1 : c;
-    if (k.kind == KIND_PAIR)
+    const bool pair = k.kind == KIND_PAIR || k.kind == KIND_BLOCH;
+    for (int c = 0; c < 6; ++c) {
+        m.o[c] = (pair || k.kind == KIND_PAIRS) ? -1 : c;
+        m.cst[c] = 0.0;
+    }
+    if (pair)
         for (int v = 0; v < ds->pd.nv; ++v) m.o[v] = 2 * ds->pd.vaxis[v];
+    if (k.kind == KIND_BLOCH)
+        for (int v = 0; v < ds->pd.nv; ++v) m.cst[v] = ds->jr0[v];
     if (k.kind == KIND_PAIRS)
         for (int v = 0; v < ds->pd.nv; ++v) { m.o[2 * v] = 2 * ds->pd.vaxis[v]; m.o[2 * v + 1] = 2 * ds->pd.vaxis[v] + 1; }
     return m;
@@ -2070,6 +2346,14 @@ static void dispatch(const LaunchKey& k, F&& f) {
             PAIR(3, 0, 2) PAIR(4, 0, 2) PAIR(3, 0, 3) PAIR(4, 0, 3) PAIR(3, 2, 2)
 #undef PAIR
             break;
+        case KIND_BLOCH:
+#define BLOCH(R, X, V)                                                                                   \
+    if (k.a == R && k.b == X && k.c == V)                                                                \
+        return go(k_bloch2<R, X, V, double>, k_bloch2_batch<R, X, V, double>, k_bloch2<R, X, V, float>,  \
+                  k_bloch2_batch<R, X, V, float>);
+            BLOCH(3, 0, 2) BLOCH(4, 0, 2) BLOCH(3, 0, 3) BLOCH(4, 0, 3) BLOCH(3, 2, 2)
+#undef BLOCH
+            break;
         case KIND_PAIRS:
 #define PAIRS(R, X, V)                                                                                   \
     if (k.a == R && k.b == X && k.c == V)                                                                \
@@ -2109,14 +2393,16 @@ static void dispatch(const LaunchKey& k, F&& f) {
     throw std::runtime_error("no compiled kernel for this launch key");
 }
 
-static void ensure_jpart(DeviceSolve* ds, int nwaves, int C) {
+// Grows the current-partial buffer on the solve's stream; returns true when
+// it was (re)allocated.
+static bool ensure_jpart(DeviceSolve* ds, int nwaves, int C) {
     size_t need = sizeof(double) * C * (size_t)ds->d.nt * nwaves;
-    if (need > ds->jpart_bytes) {
-        if (ds->jpart) HIP_CHECK(hipFreeAsync(ds->jpart, ds->stream));
-        ds->jpart = nullptr;
-        ds->jpart = dalloc(need, ds->stream);
-        ds->jpart_bytes = need;
-    }
+    if (need <= ds->jpart_bytes) return false;
+    if (ds->jpart) HIP_CHECK(hipFreeAsync(ds->jpart, ds->stream));
+    ds->jpart = nullptr;
+    ds->jpart = dalloc(need, ds->stream);
+    ds->jpart_bytes = need;
+    return true;
 }
 
 static InstPtrs ptrs_of(const DeviceSolve* ds, int nwaves, int kpw) {
@@ -2193,6 +2479,7 @@ static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
     if (out[0].kind == KIND_SEP)
         for (int w = 1; w <= 8; ++w) add(key_sep(ds, fp32, w));
     if (ds->pd.nrep > 0 && ds->use_herm) {
+        add(key_bloch(ds, fp32));
         add(key_pair(ds, fp32));
         add(key_pairs(ds, fp32));
         add(key_thread(ds, fp32));
@@ -2292,8 +2579,6 @@ double dev_run(DeviceSolve* ds, int precision, double* J_out) {
 // workgroups, longest solves first. Different keys run on separate streams
 // and overlap on the device.
 struct Batch {
-    std::vector<DeviceSolve*> items;
-    std::vector<int> prec;
     std::vector<hipStream_t> streams;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     // device tables per launch group (instances, tiles, reduction tiles),
@@ -2321,22 +2606,21 @@ Batch* batch_create() {
     return b;
 }
 void batch_destroy(Batch* b) { delete b; }
-void batch_add(Batch* b, DeviceSolve* ds, int precision) {
-    for (auto* x : b->items)
-        if (x == ds) throw std::runtime_error("solve already in this batch");
-    b->items.push_back(ds);
-    b->prec.push_back(precision);
-}
-int batch_size(Batch* b) { return (int)b->items.size(); }
 
-double batch_run(Batch* b, const std::vector<double*>& J_out) {
-    if (b->items.empty()) return 0.0;
+double batch_run(Batch* b, const std::vector<DeviceSolve*>& items, const std::vector<int>& prec,
+                 const std::vector<double*>& J_out) {
+    if (items.empty()) return 0.0;
+    if (prec.size() != items.size() || J_out.size() != items.size())
+        throw std::runtime_error("batch_run: members, precisions and outputs differ in number");
+    for (size_t i = 0; i < items.size(); ++i)
+        for (size_t j = 0; j < i; ++j)
+            if (items[i] == items[j]) throw std::runtime_error("batch_run: a solve appears twice");
     std::map<LaunchKey, std::vector<int>> groups;
-    std::vector<LaunchKey> keys(b->items.size());
-    for (size_t i = 0; i < b->items.size(); ++i)
-        if (g_autotune && !b->items[i]->tuned[b->prec[i] != 0]) dev_autotune(b->items[i], b->prec[i], 32);
-    for (size_t i = 0; i < b->items.size(); ++i) {
-        keys[i] = classify(b->items[i], b->prec[i]);
+    std::vector<LaunchKey> keys(items.size());
+    for (size_t i = 0; i < items.size(); ++i)
+        if (g_autotune && !items[i]->tuned[prec[i] != 0]) dev_autotune(items[i], prec[i], 32);
+    for (size_t i = 0; i < items.size(); ++i) {
+        keys[i] = classify(items[i], prec[i]);
         groups[keys[i]].push_back((int)i);
     }
     while (b->streams.size() < groups.size()) {
@@ -2345,7 +2629,7 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
         b->streams.push_back(s);
     }
     // finish any pending single-solve work before the batch reads the states
-    for (auto* ds : b->items) HIP_CHECK(hipStreamSynchronize(ds->stream));
+    for (auto* ds : items) HIP_CHECK(hipStreamSynchronize(ds->stream));
     // table buffers: slot 3 g + {0, 1, 2} of group g; a buffer that is too
     // small is replaced (its previous run has completed: the last run ended
     // with a synchronisation of stream 0, which waited for every group)
@@ -2376,15 +2660,17 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
         std::vector<int> order(idx.size());
         for (size_t j = 0; j < idx.size(); ++j) order[j] = (int)j;
         std::vector<double> cost(idx.size());
-        for (size_t j = 0; j < idx.size(); ++j) cost[j] = (double)b->items[idx[j]]->d.nt;
+        for (size_t j = 0; j < idx.size(); ++j) cost[j] = (double)items[idx[j]]->d.nt;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
         std::vector<int> ntiles(idx.size());
         for (size_t j = 0; j < idx.size(); ++j) {
-            DeviceSolve* ds = b->items[idx[j]];
+            DeviceSolve* ds = items[idx[j]];
             int wpt = 0;
             ntiles[j] = tiles_of(ds->d, key, wpt, block, true);
             const int nwaves = ntiles[j] * wpt;
-            ensure_jpart(ds, nwaves, comps_of(key));
+            // a partial buffer (re)allocated on the solve's stream must exist
+            // before this group's stream uses it
+            if (ensure_jpart(ds, nwaves, comps_of(key))) HIP_CHECK(hipStreamSynchronize(ds->stream));
             DevInst& I = insts[j];
             I.d = ds->d;
             I.pd = ds->pd;
@@ -2392,7 +2678,10 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
             I.J = ds->J;
             I.C = comps_of(key);
             const OMap om = omap_of(ds, key);
-            for (int c = 0; c < 6; ++c) I.omap[c] = om.o[c];
+            for (int c = 0; c < 6; ++c) {
+                I.omap[c] = om.o[c];
+                I.jcst[c] = om.cst[c];
+            }
             lds = std::max(lds, lds_of(ds->d, key));
             for (int t = 0; t < ds->d.nt; ++t) rt.push_back(make_int2((int)j, t));
         }
@@ -2434,8 +2723,8 @@ double batch_run(Batch* b, const std::vector<double*>& J_out) {
         HIP_CHECK(hipEventDestroy(ev));
     }
     HIP_CHECK(hipEventRecord(b->e1, b->streams[0]));
-    for (size_t i = 0; i < b->items.size(); ++i) {
-        DeviceSolve* ds = b->items[i];
+    for (size_t i = 0; i < items.size(); ++i) {
+        DeviceSolve* ds = items[i];
         std::swap(ds->rho, ds->rho_tmp);
         HIP_CHECK(hipMemcpyAsync(J_out[i], ds->J, sizeof(double) * 6 * (size_t)ds->d.nt, hipMemcpyDeviceToHost,
                                  b->streams[0]));
@@ -2463,6 +2752,13 @@ int dev_kernel_variant(DeviceSolve* ds, int precision) { return classify(ds, pre
 double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
     const int kind = dev_kernel_variant(ds, precision);
+    if (kind == KIND_BLOCH) {
+        // per pair 3 Bloch outputs x 2 FMAs for g and each rbc slot, stage
+        // fields 18 nx, four cross products (4 x 9), stage inputs 18, RK4
+        // accumulation 15, update 6, current nv (12 nrep + 5)
+        const PairDesc& pd = ds->pd;
+        return 12.0 * pd.nrep * (1 + pd.nx) + 18.0 * pd.nx + 36 + 18 + 15 + 6 + pd.nv * (12.0 * pd.nrep + 5);
+    }
     if (kind == KIND_PAIR || kind == KIND_PAIRS) {
         const PairDesc& pd = ds->pd;
         const double jcost = kind == KIND_PAIR ? pd.nv * (16.0 * pd.nrep + 7) : pd.nv * (32.0 * pd.nrep + 14);
@@ -2488,6 +2784,53 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     // element kernel: one nb^3 product per stage for Hermitian models, two otherwise
     const double prods = (kind == KIND_ELEM && d.hermhx) ? 4.0 : 8.0;
     return 8.0 * (prods * nb * nb * nb + 4 * d.ns * nb * nb + 16.0 * nv * nb * nb) + 6.0 * d.ns;
+}
+
+// Arithmetic per k-point and RK4 step that the model fixes, independent of
+// the kernel that runs it (flops; an FMA counts 2). Live rbc axes nx are those
+// with a non-zero field component and a non-zero block, live current axes nv
+// those with a velocity block in the window.
+//  2 bands, Hermitian blocks (Bloch vector s, ds/dt = 2 h x s): per +-R pair
+//    3 outputs x 2 FMAs for h and each live rbc axis (12 (1 + nx) nrep),
+//    stage fields 18 nx, four cross products 36, stage inputs 18, RK4
+//    accumulation 15, update 6, current per axis 12 nrep + 5 for a Hermitian
+//    window (twice that for a complex one).
+//  3+ bands, Hermitian H and rbc: the Fourier sums of H and of each live rbc
+//    axis over the upper triangle with +-R pairs folded, per distinct n2 value
+//    of the support in the row-separable form (4 nd nb^2 (1 + nx)), stage
+//    fields 6 nx nb^2, per stage K = -i (M - M^dagger) with M = H X as three
+//    real products (4 x 6 nb^3), M - M^dagger, stage inputs and RK4 update
+//    14 nb^2, current per axis and distinct window n2 value 4 nb^2 + 8 (the
+//    Hermitian rho folds each element pair into two real x complex MACs).
+//  Otherwise (non-Hermitian blocks): two complex products per stage, direct
+//    Fourier sums over the ns supports and the window.
+double dev_min_flops_per_kstep(DeviceSolve* ds) {
+    const SolveDesc& d = ds->d;
+    const PairDesc& pd = ds->pd;
+    const double nb = d.nb, nb2 = nb * nb;
+    int nx = 0, nv = 0;
+    for (int a = 0; a < 3; ++a) { nx += (d.xmask >> a) & 1; nv += (d.vmask >> a) & 1; }
+    if (d.nb == 2 && pd.nrep > 0) {
+        const double cur = pd.nv * (12.0 * pd.nrep + 5) * (pd.vpaired ? 1 : 2);
+        return 12.0 * pd.nrep * (1 + pd.nx) + 18.0 * pd.nx + 36 + 18 + 15 + 6 + cur;
+    }
+    auto distinct_n2 = [&](uint32_t mask) {
+        std::vector<int> v;
+        for (int s = 0; s < d.ns; ++s)
+            if ((d.opmask[s] & mask) && std::find(v.begin(), v.end(), d.n2[s]) == v.end()) v.push_back(d.n2[s]);
+        return (double)v.size();
+    };
+    const uint32_t hx = 1u | (d.xmask << 1), vm = (uint32_t)d.vmask << 4;
+    if (d.nb >= 3 && d.hermhx) {
+        const double nd = distinct_n2(hx), nwd = distinct_n2(vm);
+        return 4.0 * nd * nb2 * (1 + nx) + 6.0 * nx * nb2 + 24.0 * nb2 * nb + 14.0 * nb2 + nv * nwd * (4.0 * nb2 + 8);
+    }
+    int nsup = 0, nwin = 0;
+    for (int s = 0; s < d.ns; ++s) {
+        nsup += (d.opmask[s] & hx) != 0;
+        nwin += (d.opmask[s] & vm) != 0;
+    }
+    return 8.0 * (8 * nb2 * nb + (1 + nx) * nsup * nb2 + nv * nwin * nb2) + 24.0 * nx * nb2 + 16.0 * nb2;
 }
 
 double dev_last_kernel_ms(DeviceSolve* ds) { return ds ? ds->last_kernel_ms : 0.0; }

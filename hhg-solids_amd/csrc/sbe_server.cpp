@@ -68,6 +68,7 @@ struct Server::Impl {
     std::vector<std::shared_ptr<ServerRequest>> inflight;
     long long next_id = 1;
     size_t bytes_in_use = 0;
+    int preparing = 0;  // requests a worker has taken and not yet handed on
     bool stop = false;
     std::vector<std::thread> workers;
     std::thread scheduler;
@@ -83,6 +84,7 @@ struct Server::Impl {
                 r = to_prepare.front();
                 to_prepare.pop_front();
                 r->state = REQ_PREPARING;
+                ++preparing;
             }
             std::unique_ptr<HostSolve> hs;
             std::string err;
@@ -97,6 +99,7 @@ struct Server::Impl {
             }
             {
                 std::lock_guard<std::mutex> lk(mu);
+                --preparing;
                 r->t_prepared = now_s();
                 if (!err.empty()) {
                     r->state = REQ_FAILED;
@@ -109,8 +112,8 @@ struct Server::Impl {
                     r->bytes = bytes;
                     r->state = REQ_PREPARED;
                     prepared.push_back(r);
-                    cv_sched.notify_all();
                 }
+                cv_sched.notify_all();  // new work, or the last preparation of a shutdown ended
             }
         }
     }
@@ -121,9 +124,13 @@ struct Server::Impl {
             std::vector<std::shared_ptr<ServerRequest>> admit;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                if (inflight.empty())
-                    cv_sched.wait(lk, [&] { return stop || !prepared.empty(); });
-                if (stop && prepared.empty() && inflight.empty() && to_prepare.empty()) return;
+                // the server is drained when nothing is queued, being prepared,
+                // prepared or on the device
+                auto drained = [&] {
+                    return stop && prepared.empty() && inflight.empty() && to_prepare.empty() && preparing == 0;
+                };
+                if (inflight.empty()) cv_sched.wait(lk, [&] { return drained() || !prepared.empty(); });
+                if (drained()) return;
                 size_t used = bytes_in_use;
                 int slots = cfg.max_inflight - (int)inflight.size();
                 if (cfg.policy == 1 && prepared.size() > 1) {

@@ -101,6 +101,7 @@ struct DevInst {
     double* J;           // [nt][6] reduced current
     int C;               // current components per wave partial
     int omap[6];         // output slot (re/im of x, y, z = 0..5) of each component, -1 = unused
+    double jcst[6];      // constant added to each component after the 1/N normalisation
 };
 
 }  // namespace hhg

@@ -8,6 +8,7 @@
 // (src/swe.cpp:235-242), restart rebuilds from the file (src/swe.cpp:18-35).
 // Unlike the reference, no exception or exit crosses the ABI: errors are
 // recorded on the handle and in a thread-local last-error string.
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "hhg_swe.h"
 #include "sbe_host.h"
 #include "sbe_server.h"
 
@@ -38,6 +40,7 @@ struct Settings {
 
 struct SWESim {
     std::string path_tb;
+    std::vector<hhg_batch*> batches;  // batches this solve is a member of
     Settings* settings = nullptr;
     int precision = 0;
     int variant = 0;
@@ -51,6 +54,21 @@ struct SWESim {
     ~SWESim() {
         hhg::dev_destroy(dev);
         hhg::fftform_destroy(ff);
+    }
+};
+
+// A batch names its members by handle; it resolves their device solves and
+// precisions when it runs, so a member's init, restart or set_precision after
+// the add is seen by the next batch run, and deleting a member removes it
+// from its batches.
+struct hhg_batch {
+    hhg::Batch* b = nullptr;
+    std::vector<SWESim*> sims;
+    std::string err;
+    double last_ms = 0.0;
+    ~hhg_batch() {
+        for (SWESim* s : sims) s->batches.erase(std::find(s->batches.begin(), s->batches.end(), this));
+        hhg::batch_destroy(b);
     }
 };
 
@@ -169,7 +187,29 @@ API void SWESim_get_current(SWESim* s, double* t, cplx* jx, cplx* jy, cplx* jz) 
     });
 }
 
-API void SWESim_delete(SWESim* s) { delete s; }
+API int SWESim_get_current_run(SWESim* s, int run, double* t, cplx* jx, cplx* jy, cplx* jz) {
+    guarded(s, [&] {
+        if (!s->host) throw std::runtime_error("SWESim_get_current_run: solve not initialised");
+        const int nt = s->host->d.nt;
+        const int runs = nt > 0 ? (int)(s->J[0].size() / nt) : 0;
+        if (run < 0) run += runs;
+        if (run < 0 || run >= runs)
+            throw std::runtime_error("SWESim_get_current_run: run " + std::to_string(run) + " of " + std::to_string(runs));
+        cplx* out[3] = {jx, jy, jz};
+        for (int ti = 0; ti < nt; ++ti) {
+            if (t) t[ti] = s->host->t[ti];
+            for (int a = 0; a < 3; ++a)
+                if (out[a]) out[a][ti] = s->J[a][(size_t)run * nt + ti];
+        }
+    });
+    return s ? s->status : -1;
+}
+
+API void SWESim_delete(SWESim* s) {
+    if (!s) return;
+    for (hhg_batch* b : s->batches) b->sims.erase(std::find(b->sims.begin(), b->sims.end(), s));
+    delete s;
+}
 
 API Settings* Settings_new() {
     try { return new Settings(); } catch (...) { g_last_error = "allocation failed"; return nullptr; }
@@ -235,6 +275,9 @@ API void SWESim_set_kernel_variant(SWESim* s, int v) {
 }
 API double SWESim_flops_per_kstep(SWESim* s) {
     return (s && s->dev) ? hhg::dev_flops_per_kstep(s->dev, s->precision) : 0.0;
+}
+API double SWESim_min_flops_per_kstep(SWESim* s) {
+    return (s && s->dev) ? hhg::dev_min_flops_per_kstep(s->dev) : 0.0;
 }
 API int SWESim_get_kernel(SWESim* s) {
     if (s && s->dev && s->variant == 2) return 7;
@@ -311,12 +354,6 @@ API double SWESim_last_kernel_ms(SWESim* s) { return (s && s->dev) ? hhg::dev_la
 }  // extern "C"
 
 // ---- batches -----------------------------------------------------------
-struct hhg_batch {
-    hhg::Batch* b = nullptr;
-    std::vector<SWESim*> sims;
-    std::string err;
-    ~hhg_batch() { hhg::batch_destroy(b); }
-};
 
 template <typename F>
 static int bguard(hhg_batch* b, F&& f) {
@@ -349,8 +386,10 @@ API int hhg_batch_add(hhg_batch* b, SWESim* s) {
     return bguard(b, [&] {
         if (!s || !s->dev) throw std::runtime_error("hhg_batch_add: solve not initialised on the device");
         if (s->variant == 2) throw std::runtime_error("hhg_batch_add: the FFT form (kernel variant 2) runs alone");
-        hhg::batch_add(b->b, s->dev, s->precision);
+        if (std::find(b->sims.begin(), b->sims.end(), s) != b->sims.end())
+            throw std::runtime_error("hhg_batch_add: solve already in this batch");
         b->sims.push_back(s);
+        s->batches.push_back(b);
     });
 }
 API int hhg_batch_size(hhg_batch* b) { return b ? (int)b->sims.size() : 0; }
@@ -358,11 +397,21 @@ API int hhg_batch_run(hhg_batch* b) {
     return bguard(b, [&] {
         std::vector<std::vector<double>> J(b->sims.size());
         std::vector<double*> out(b->sims.size());
+        std::vector<hhg::DeviceSolve*> items(b->sims.size());
+        std::vector<int> prec(b->sims.size());
         for (size_t i = 0; i < b->sims.size(); ++i) {
-            J[i].resize((size_t)6 * b->sims[i]->host->d.nt);
+            SWESim* s = b->sims[i];
+            if (!s->dev || !s->host)
+                throw std::runtime_error("hhg_batch_run: member " + std::to_string(i) + " is not initialised on the device");
+            if (s->variant == 2)
+                throw std::runtime_error("hhg_batch_run: member " + std::to_string(i) + " uses the FFT form (kernel variant 2), which runs alone");
+            items[i] = s->dev;
+            prec[i] = s->precision;
+            J[i].resize((size_t)6 * s->host->d.nt);
             out[i] = J[i].data();
         }
-        const double ms = hhg::batch_run(b->b, out);
+        const double ms = hhg::batch_run(b->b, items, prec, out);
+        b->last_ms = ms;
         for (size_t i = 0; i < b->sims.size(); ++i) {
             SWESim* s = b->sims[i];
             const int nt = s->host->d.nt;
@@ -471,7 +520,7 @@ API const char* hhg_server_error(hhg_server* s, long long id) {
     return msg.c_str();
 }
 
-API double hhg_batch_last_ms(hhg_batch* b) { return (b && !b->sims.empty()) ? b->sims[0]->last_ms : 0.0; }
+API double hhg_batch_last_ms(hhg_batch* b) { return b ? b->last_ms : 0.0; }
 API const char* hhg_batch_get_error(hhg_batch* b) { return b ? b->err.c_str() : "null batch handle"; }
 
 }  // extern "C"

@@ -42,6 +42,7 @@ def bind(lib):
         "SWESim_set_path_tb": ([V, P], None), "SWESim_set_settings": ([V, V], None),
         "SWESim_init": ([V], None), "SWESim_restart": ([V], None), "SWESim_init_host": ([V], None),
         "SWESim_get_current": ([V, _D, _C, _C, _C], None), "SWESim_delete": ([V], None),
+        "SWESim_get_current_run": ([V, I, _D, _C, _C, _C], I),
         "Settings_new": ([], V), "Settings_delete": ([V], None),
         "Settings_set_nr1": ([V, I], None), "Settings_set_nr2": ([V, I], None), "Settings_set_nr3": ([V, I], None),
         "Settings_set_tmax": ([V, Dd], None), "Settings_set_dt": ([V, Dd], None),
@@ -53,7 +54,8 @@ def bind(lib):
         "SWESim_get_fields": ([V, _D, _D, _D], I), "SWESim_get_rho_k": ([V, _C], I),
         "SWESim_last_run_ms": ([V], Dd), "SWESim_last_kernel_ms": ([V], Dd),
         "SWESim_set_kernel_variant": ([V, I], None), "SWESim_get_kernel": ([V], I),
-        "SWESim_flops_per_kstep": ([V], Dd), "SWESim_autotune": ([V, I], I),
+        "SWESim_flops_per_kstep": ([V], Dd), "SWESim_min_flops_per_kstep": ([V], Dd),
+        "SWESim_autotune": ([V, I], I),
         "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
         "SWESim_get_kpoints_per_wave": ([V], I),
         "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
@@ -237,7 +239,12 @@ class Solve:
         return self.lib.SWESim_get_kernel(self.ptr)
 
     def flops_per_kstep(self):
+        """Flops per k-point and RK4 step of the kernel the next run launches."""
         return self.lib.SWESim_flops_per_kstep(self.ptr)
+
+    def min_flops_per_kstep(self):
+        """Flops per k-point and RK4 step that the model fixes, independent of the kernel."""
+        return self.lib.SWESim_min_flops_per_kstep(self.ptr)
 
     def autotune(self, steps=32):
         """Time the launch configurations that cover this solve on its first
@@ -268,6 +275,14 @@ class Solve:
         J = [np.zeros(self.nt, dtype=np.complex128) for _ in range(3)]
         self.lib.SWESim_get_current(self.ptr, t, *J)
         self._check("get_current")
+        return t, np.stack(J, axis=1)
+
+    def current_run(self, run=-1):
+        """J(t) of one run (0 = first, -1 = last): shape (nt, 3)."""
+        t = np.zeros(self.nt)
+        J = [np.zeros(self.nt, dtype=np.complex128) for _ in range(3)]
+        self.lib.SWESim_get_current_run(self.ptr, int(run), t, *J)
+        self._check("get_current_run")
         return t, np.stack(J, axis=1)
 
     def fields(self):

@@ -116,11 +116,12 @@ class Server:
         self.close()
 
 
-def run_stream(server, arrivals, keep_results=False):
+def run_stream(server, arrivals, keep_results=False, keep_first=None):
     """Submit ``[(t_arrival_s, param)]`` at their arrival times (relative to
     now), wait for all, and return per-request records and a summary. With
-    ``keep_results`` each completed record also holds its ``t`` and ``J``
-    (fetched after the wait; the latencies come from the server's clock)."""
+    ``keep_results`` each completed record (only the first ``keep_first`` when
+    given) also holds its ``t`` and ``J`` (fetched after the wait; the
+    latencies come from the server's clock)."""
     ids = []
     t0 = time.perf_counter()
     for ta, p in arrivals:
@@ -136,7 +137,7 @@ def run_stream(server, arrivals, keep_results=False):
                      "queue_s": tm["admitted"] - tm["submit"] if tm["admitted"] > 0 else None,
                      "device_ms": tm["device_ms"], "ksteps": tm["ksteps"], "nb_grid": (p["nr1"], p["nr2"]),
                      "t_done": tm["done"], "t_submit": tm["submit"]})
-        if keep_results and st == "done":
+        if keep_results and st == "done" and (keep_first is None or len(recs) <= keep_first):
             recs[-1]["t"], recs[-1]["J"] = server.result(rid)
     ok = [r for r in recs if r["state"] == "done"]
     lat = np.array([r["latency_s"] for r in ok]) if ok else np.zeros(1)

@@ -13,13 +13,19 @@
 #ifndef HHG_SWE_H
 #define HHG_SWE_H
 
+/* cdouble of src/cwrapper.cpp: std::complex<double> in C++ (the library's
+ * own definitions are compiled against these declarations), a layout-
+ * identical pair of doubles in C. */
 #ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> hhg_cdouble;
 extern "C" {
+#else
+typedef struct { double re, im; } hhg_cdouble;
 #endif
 
 typedef struct SWESim SWESim;
 typedef struct Settings Settings;
-typedef struct { double re, im; } hhg_cdouble;  /* = std::complex<double> */
 
 /* ---- reference ABI (src/cwrapper.cpp) ---------------------------------- */
 SWESim* SWESim_new(void);                                   /* cwrapper.cpp:7  */
@@ -58,6 +64,10 @@ int SWESim_get_status(SWESim* swe);
 const char* SWESim_get_error(SWESim* swe);
 /* Message of the last failed call in this thread, on any handle. */
 const char* hhg_last_error(void);
+/* Current of one run: SWESim_get_current returns the first nt samples (the
+ * first run, src/swe.cpp:235-242); this returns run `run` (0 = first, -1 =
+ * the last one). Returns 0 on success. */
+int SWESim_get_current_run(SWESim* swe, int run, double* t, hhg_cdouble* jx, hhg_cdouble* jy, hhg_cdouble* jz);
 /* Number of time steps of the initialised solve (= (int)(tmax/dt)). */
 int SWESim_get_nt(SWESim* swe);
 int SWESim_get_num_orb(SWESim* swe);
@@ -73,8 +83,9 @@ void SWESim_set_precision(SWESim* swe, int precision);
 void SWESim_set_kernel_variant(SWESim* swe, int variant);
 /* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2 with
  * register pairs, 3 = element per lane, 4 = Hermitian nb=2 with streamed
- * pairs, 6 = row-separable (Hermitian nb >= 3), 7 = FFT form (variant 2);
- * 0 before SWESim_init. */
+ * pairs, 6 = row-separable (Hermitian nb >= 3), 7 = FFT form (variant 2),
+ * 8 = Hermitian nb=2 with register pairs in Bloch-vector form (every k-point
+ * has the same occupation); 0 before SWESim_init. */
 int SWESim_get_kernel(SWESim* swe);
 /* Autotuner: times every launch configuration that covers this solve (kernel
  * family, workgroup size) on its first `steps` RK4 steps without changing
@@ -117,6 +128,12 @@ void hhg_fft2_plan_destroy(hhg_fft2_plan* plan);
 /* Arithmetic of the launched formulation per k-point and RK4 step (flops,
  * a fused multiply-add counted as 2). */
 double SWESim_flops_per_kstep(SWESim* swe);
+/* Arithmetic per k-point and RK4 step that the model itself fixes,
+ * independent of the kernel: Hermitian +-R pairs folded, only the live rbc
+ * axes (non-zero field component) and current axes, the 2-band dynamics as a
+ * Bloch-vector cross product, 3+ bands as K = -i (M - M^dagger) with M = H X
+ * in three real products. The reference for roofline fractions. */
+double SWESim_min_flops_per_kstep(SWESim* swe);
 /* Field tables exactly as the time loop consumes them: t[nt] (au),
  * E[3][nt], A[3][nt]. Returns 0 on success. */
 int SWESim_get_fields(SWESim* swe, double* t, double* E, double* A);

@@ -42,6 +42,8 @@ def lib():
         L.oracle_rho0.argtypes = [ct.c_char_p, _I, _D, ct.c_int, _D]
         L.oracle_velocity_window.argtypes = [ct.c_char_p, _I, _D, ct.c_int, _D]
         L.oracle_run.argtypes = [ct.c_char_p, _I, _D, ct.c_int, ct.c_int, _D, _D]
+        L.oracle_run_from.argtypes = [ct.c_char_p, _I, _D, _D, ct.c_int, _D, _D]
+        L.oracle_propagate_k.argtypes = [ct.c_char_p, _I, _D, ct.c_int, _I, ct.c_int, _D]
         L.oracle_fft2.argtypes = [_D, ct.c_int, ct.c_int, ct.c_int]
         L.oracle_eigh.argtypes = [ct.c_int, _D, _D, _D]
         L.oracle_equilibrium_rho.argtypes = [ct.c_int, _D, _D]
@@ -132,6 +134,35 @@ def run(path, prm, mode="perk", nthreads=None):
     J = np.zeros(6 * n)
     _check(lib().oracle_run(path.encode(), nr, dp, 0 if mode == "perk" else 1, int(nthreads), t, J))
     return t, J.view(np.complex128).reshape(n, 3)
+
+
+def run_from(path, prm, rho_init, nthreads=None):
+    """Per-k solve started from the states rho_init (N, nb, nb) instead of the
+    initial state (a continued run). Returns t (nt,) and J (nt, 3)."""
+    n = nt(prm)
+    nr, dp = pack(prm)
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    rho = np.ascontiguousarray(rho_init, dtype=np.complex128)
+    if rho.shape[0] != int(nr[0]) * int(nr[1]):
+        raise ValueError("rho_init must hold one state per k-point")
+    t = np.zeros(n)
+    J = np.zeros(6 * n)
+    _check(lib().oracle_run_from(path.encode(), nr, dp, rho.view(np.float64).reshape(-1), int(nthreads), t, J))
+    return t, J.view(np.complex128).reshape(n, 3)
+
+
+def propagate_k(path, prm, kidx, nthreads=None):
+    """Final rho_k (after all nt RK4 steps, per-k form) of the listed k-points
+    k = m1*nr2 + m2: shape (len(kidx), nb, nb) complex128."""
+    nb, _ = model_dims(path)
+    nr, dp = pack(prm)
+    kidx = np.ascontiguousarray(kidx, dtype=np.int32)
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1, max(1, len(kidx)))
+    out = np.zeros(2 * len(kidx) * nb * nb)
+    _check(lib().oracle_propagate_k(path.encode(), nr, dp, len(kidx), kidx, int(nthreads), out))
+    return out.view(np.complex128).reshape(len(kidx), nb, nb)
 
 
 def fft2(a, sign=-1):

@@ -672,7 +672,11 @@ static void perk_hk(const PerK& P, int m1, int m2, cd* Hk) {
     }
 }
 
-static void perk_range(const PerK& P, int k0, int k1, std::vector<cd>& J /* nt*3 */) {
+// Propagates k-points kidx[0..nk) (k = m1 N2 + m2) over all nt steps; adds
+// their current terms to J (nt*3, unnormalised) and, when rho_out is given,
+// stores each one's final rho_k at rho_out[i*nb*nb].
+static void perk_list(const PerK& P, const int* kidx, int nk, std::vector<cd>& J, cd* rho_out,
+                      const cd* rho_init = nullptr /* [N][nb*nb]: start from these states */) {
     const int nb = P.nb, nb2 = nb * nb, nt = P.nt, ns = (int)P.sn1.size(), nw = (int)P.wn1.size();
     std::vector<cd> Hk(nb2), rho(nb2), Ht(nb2), Xt[3], Hs(nb2), Kc[4], rs(nb2), ws(ns), ww(nw), w0s(ns), w0w(nw);
     for (int a = 0; a < 3; ++a) Xt[a].resize(nb2);
@@ -689,10 +693,15 @@ static void perk_range(const PerK& P, int k0, int k1, std::vector<cd>& J /* nt*3
     auto heff = [&](const double* e) {
         for (int q = 0; q < nb2; ++q) Hs[q] = Ht[q] + e[0] * Xt[0][q] + e[1] * Xt[1][q] + e[2] * Xt[2][q];
     };
-    for (int k = k0; k < k1; ++k) {
+    for (int ik = 0; ik < nk; ++ik) {
+        const int k = kidx[ik];
         int m1 = k / P.N2, m2 = k % P.N2;
-        perk_hk(P, m1, m2, Hk.data());
-        equilibrium_rho(nb, Hk.data(), rho.data(), nullptr);
+        if (rho_init) {
+            std::copy(rho_init + (size_t)k * nb2, rho_init + (size_t)(k + 1) * nb2, rho.begin());
+        } else {
+            perk_hk(P, m1, m2, Hk.data());
+            equilibrium_rho(nb, Hk.data(), rho.data(), nullptr);
+        }
         for (int s = 0; s < ns; ++s) w0s[s] = P.tw1[imod((long)m1 * P.sn1[s], P.N1)] * P.tw2[imod((long)m2 * P.sn2[s], P.N2)];
         for (int s = 0; s < nw; ++s) w0w[s] = P.tw1[imod((long)m1 * P.wn1[s], P.N1)] * P.tw2[imod((long)m2 * P.wn2[s], P.N2)];
         for (int ti = 0; ti < nt; ++ti) {
@@ -727,17 +736,27 @@ static void perk_range(const PerK& P, int k0, int k1, std::vector<cd>& J /* nt*3
                 J[(size_t)ti * 3 + a] += sum;
             }
         }
+        if (rho_out) std::copy(rho.begin(), rho.end(), rho_out + (size_t)ik * nb2);
     }
 }
 
-static void run_perk(const TB& tb, const Params& p, int nthreads, double* t_out, double* J_out) {
+static void perk_range(const PerK& P, int k0, int k1, std::vector<cd>& J /* nt*3 */, const cd* rho_init) {
+    std::vector<int> ks;
+    for (int k = k0; k < k1; ++k) ks.push_back(k);
+    perk_list(P, ks.data(), (int)ks.size(), J, nullptr, rho_init);
+}
+
+// rho_init: null = the T = 0 initial state; else [N][nb*nb] states to start
+// from (a run that continues a previous one, src/swe.cpp:183-223 called twice)
+static void run_perk(const TB& tb, const Params& p, int nthreads, double* t_out, double* J_out,
+                     const cd* rho_init = nullptr) {
     PerK P = setup_perk(tb, p);
     if (nthreads < 1) nthreads = 1;
     std::vector<std::vector<cd>> Jp(nthreads, std::vector<cd>((size_t)P.nt * 3, 0.0));
     std::vector<std::thread> th;
     for (int t = 0; t < nthreads; ++t) {
         int k0 = (int)((long)P.N * t / nthreads), k1 = (int)((long)P.N * (t + 1) / nthreads);
-        th.emplace_back(perk_range, std::cref(P), k0, k1, std::ref(Jp[t]));
+        th.emplace_back(perk_range, std::cref(P), k0, k1, std::ref(Jp[t]), rho_init);
     }
     for (auto& x : th) x.join();
     for (int ti = 0; ti < P.nt; ++ti)
@@ -843,6 +862,37 @@ API int oracle_run(const char* path, const int* nr, const double* dp, int mode, 
         TB tb = parse_tb(path); Params p = make_params(nr, dp);
         if (mode == 0) run_perk(tb, p, nthreads, t_out, J_out);
         else run_fftform(tb, p, t_out, J_out);
+    )
+}
+
+// Per-k solve that starts from the given states rho_init[N][nb*nb] (complex)
+// instead of the initial state: the J(t) of a second run_simulation.
+API int oracle_run_from(const char* path, const int* nr, const double* dp, const double* rho_init, int nthreads,
+                        double* t_out, double* J_out) {
+    GUARD(
+        TB tb = parse_tb(path); Params p = make_params(nr, dp);
+        run_perk(tb, p, nthreads, t_out, J_out, reinterpret_cast<const cd*>(rho_init));
+    )
+}
+
+// Final rho_k (after all nt steps, per-k form) of the nk listed k-points,
+// k = m1*nr2 + m2; out[nk][nb*nb] complex. nthreads threads over the list.
+API int oracle_propagate_k(const char* path, const int* nr, const double* dp, int nk, const int* kidx, int nthreads,
+                           double* out) {
+    GUARD(
+        TB tb = parse_tb(path); Params p = make_params(nr, dp); PerK P = setup_perk(tb, p);
+        for (int i = 0; i < nk; ++i)
+            if (kidx[i] < 0 || kidx[i] >= P.N) throw std::runtime_error("k index out of range");
+        if (nthreads < 1) nthreads = 1;
+        const int nb2 = P.nb * P.nb;
+        cd* o = reinterpret_cast<cd*>(out);
+        std::vector<std::thread> th;
+        std::vector<std::vector<cd>> Jp(nthreads, std::vector<cd>((size_t)P.nt * 3, 0.0));
+        for (int t = 0; t < nthreads; ++t) {
+            const int i0 = (int)((long)nk * t / nthreads), i1 = (int)((long)nk * (t + 1) / nthreads);
+            th.emplace_back([&, i0, i1, t] { perk_list(P, kidx + i0, i1 - i0, Jp[t], o + (size_t)i0 * nb2); });
+        }
+        for (auto& x : th) x.join();
     )
 }
 

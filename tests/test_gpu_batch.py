@@ -18,6 +18,11 @@ def _mixed_params(tmp_path):
         files[nb] = str(tmp_path / f"syn{nb}_tb.dat")
         tbgen.write_tb(files[nb], *tbgen.synthetic(nb, seed=20 + nb))
     short = dict(tmax=8.0, tmax_field=7.0, intensity=2e12)
+    # Haldane model shifted by -1 eV on site: two filled bands near the K points, one elsewhere
+    lat, R, H, r = tbgen.haldane()
+    H[[tuple(x[:2]) for x in R].index((0, 0))] += -1.0 * np.eye(2)
+    doped = str(tmp_path / "haldane_doped_tb.dat")
+    tbgen.write_tb(doped, lat, R, H, r)
     return [
         make_param(HALDANE, nr=(16, 20), pol=(0.6, 0.8, 0.0), **short),
         make_param(HMCASE0, nr=(12, 12), pol=(0.0, 1.0, 0.0), **short),
@@ -27,6 +32,7 @@ def _mixed_params(tmp_path):
         make_param(files[4], nr=(8, 8), pol=(0.6, 0.8, 0.0), **short),
         make_param(files[8], nr=(6, 6), pol=(0.8, 0.6, 0.0), tmax=4.0, tmax_field=3.0, intensity=2e12),
         dict(make_param(files[4], nr=(6, 8), pol=(0.6, 0.8, 0.0), **short), precision="fp32"),
+        make_param(doped, nr=(16, 20), pol=(0.6, 0.8, 0.0), **short),
     ]
 
 
@@ -40,7 +46,8 @@ def test_batch_members_equal_single_runs(tmp_path):
             alone.append(s.current()[1])
     solves = [lib.Solve(p) for p in params]
     kinds = sorted({s.kernel() for s in solves})
-    assert kinds == [2, 4, 6]  # register pairs, streamed pairs and row-separable kernels in one batch
+    # register pairs (4-component and Bloch-vector), streamed pairs and row-separable kernels in one batch
+    assert kinds == [2, 4, 6, 8]
     with lib.Batch(solves) as b:
         b.run()
     for p, s, Ja in zip(params, solves, alone):

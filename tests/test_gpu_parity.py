@@ -114,20 +114,30 @@ def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, n
     assert np.abs(np.trace(rho, axis1=1, axis2=2) - tr0).max() < 1e-10
 
 
-@pytest.mark.parametrize("model", ["hmcase0", "haldane", "synthetic2", "synthetic4", "synthetic8"])
+@pytest.mark.parametrize("model", ["hmcase0", "haldane", "haldane_doped", "synthetic2", "synthetic4", "synthetic8"])
 def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model):
-    """The Hermitian-paired nb=2 kernels (register pairs: real current;
-    streamed pairs: complex current of an asymmetric window) and the generic
-    thread-per-k kernel against the oracle on the same input."""
+    """The Hermitian-paired nb=2 kernels (Bloch vector when every k-point has
+    the same trace, register pairs otherwise: real current; streamed pairs:
+    complex current of an asymmetric window) and the generic thread-per-k
+    kernel against the oracle on the same input."""
     from hhgamd import tbgen
     if model.startswith("synthetic"):
         nb = int(model[-1])
         path = str(tmp_path / f"syn{nb}_tb.dat")
         tbgen.write_tb(path, *tbgen.synthetic(nb, seed=77))
         expect = {2: (4, 1), 4: (6, 3), 8: (6, 3)}[nb]  # streamed pairs / row-separable vs element-per-lane
+    elif model == "haldane_doped":
+        # on-site shift -1 eV: near the K points both bands lie below E_F = 0, so the
+        # occupation (trace of rho_k) is 2 there and 1 elsewhere: the 4-component pair kernel
+        lat, R, H, r = tbgen.haldane()
+        i0 = [tuple(x[:2]) for x in R].index((0, 0))
+        H[i0] += -1.0 * np.eye(2)
+        path = str(tmp_path / "haldane_doped_tb.dat")
+        tbgen.write_tb(path, lat, R, H, r)
+        expect = (2, 1)
     else:
         path = {"hmcase0": HMCASE0, "haldane": HALDANE}[model]
-        expect = (2, 1)
+        expect = (8, 1)  # Bloch-vector kernel: one filled band at every k
     nr = (16, 20) if "8" not in model else (8, 10)
     prm = make_param(path, nr=nr, tmax=30.0 if "8" not in model else 8.0, tmax_field=25.0 if "8" not in model else 7.0,
                      intensity=3e12, pol=(0.6, 0.8, 0.0))
@@ -142,6 +152,10 @@ def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model
         print(f"{model} {kern} kernel {out[kern]}: rel L2 vs oracle {err:.2e}")
         assert err <= FP64_TOL, kern
     assert out == {"auto": expect[0], "generic": expect[1]}
+    if model == "haldane_doped":
+        with native.Solve(prm, device=False) as s:
+            tr = np.trace(s.rho_k(), axis1=1, axis2=2).real
+        assert tr.min() < 1.5 < tr.max()  # the occupation differs between k-points
     if model == "synthetic2":
         assert np.abs(Jref.imag).max() > 1e-6 * np.abs(Jref).max()  # the window current is complex here
 

@@ -206,6 +206,49 @@ def test_perk_thread_count_invariance(oracle_lib):
     assert rel_l2(J5, J1) < 1e-12  # only the k-partition of the sum differs
 
 
+def window_current(path, prm, rho, ti):
+    """J_a[ti] = (1/N) sum_k sum_{n in 4x4 window} sum_ij v_a(n)_ij conj(rho_k,ij) e^{-i(k - A[ti]).R(n)}
+    from final states rho (N, nb, nb): the k-space form of src/observable.cpp:16-30
+    (v in eV*Angstrom, R in bohr, k.R(n) = 2 pi (m1 n1 / N1 + m2 n2 / N2))."""
+    import oracle
+    lat, _, _, _ = oracle.model(path)
+    _, _, A = oracle.fields(prm)
+    v = oracle.velocity_window(path, prm)  # (3, 4, 4, nb, nb), n1, n2 in {-2..1}
+    N1, N2 = prm["nr1"], prm["nr2"]
+    m1, m2 = np.divmod(np.arange(N1 * N2), N2)
+    J = np.zeros(3, dtype=complex)
+    for i1, n1 in enumerate(range(-2, 2)):
+        for i2, n2 in enumerate(range(-2, 2)):
+            R = (n1 * lat[0] + n2 * lat[1]) / BOHR
+            ph = np.exp(-2j * np.pi * (m1 * n1 / N1 + m2 * n2 / N2)) * np.exp(1j * A[:, ti] @ R)
+            for a in range(3):
+                J[a] += np.sum(ph * np.einsum("ij,kij->k", v[a, i1, i2], np.conj(rho)))
+    return J / (N1 * N2)
+
+
+def test_propagate_k_closes_the_last_current(oracle_lib, tmp_path):
+    """The final states of oracle_propagate_k over every k reproduce the last
+    current sample of oracle_run through the window's k-space closed form,
+    and stay Hermitian with their initial traces."""
+    from hhgamd import tbgen
+    path = str(tmp_path / "syn4_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(4, seed=7))
+    prm = make_param(path, nr=(6, 8), tmax=0.02197 * 24 + 1e-9, tmax_field=0.5, intensity=5e11,
+                     pol=(0.6, 0.8, 0.0))
+    nt = oracle_lib.nt(prm)
+    rho = oracle_lib.propagate_k(path, prm, np.arange(48), nthreads=3)
+    rho0 = oracle_lib.rho0(path, prm)
+    assert np.abs(rho - np.conj(np.transpose(rho, (0, 2, 1)))).max() < 1e-14
+    assert np.abs(np.trace(rho, axis1=1, axis2=2) - np.trace(rho0, axis1=1, axis2=2)).max() < 1e-13
+    assert np.abs(rho - rho0).max() > 1e-6  # the field moved the states
+    _, J = oracle_lib.run(path, prm, mode="perk", nthreads=2)
+    Jw = window_current(path, prm, rho, nt - 1)
+    assert np.abs(Jw - J[nt - 1]).max() <= 1e-12 * np.abs(J[nt - 1]).max()
+    # a subset in another order gives the same states
+    sub = np.array([47, 3, 20], dtype=np.int32)
+    assert np.array_equal(oracle_lib.propagate_k(path, prm, sub, nthreads=2), rho[sub])
+
+
 @pytest.mark.parametrize("name,path,n", [("hmcase0", HMCASE0, 10), ("hmcase0", HMCASE0, 30), ("haldane", HALDANE, 30)])
 def test_oracle_reproduces_golden(oracle_lib, name, path, n):
     g = np.load(f"{GOLDEN}/golden_{name}_{n}.npz")
