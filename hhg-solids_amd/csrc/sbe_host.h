@@ -84,6 +84,12 @@ DeviceSolve* dev_create(const HostSolve& hs);
 // Propagates nt steps from the current device state. J_out: [nt][6].
 // Returns the device time in milliseconds.
 double dev_run(DeviceSolve* ds, int precision, double* J_out);
+// The same run without allocating device memory: the current partials go to
+// the library's static scratch array on the solve's device (at most
+// static_scratch_bytes(); larger launches are refused). Concurrent calls of
+// this process are serialised.
+double dev_run_static(DeviceSolve* ds, int precision, double* J_out);
+size_t static_scratch_bytes();
 // Asynchronous form: enqueue on the solve's stream, poll, then complete.
 void dev_launch(DeviceSolve* ds, int precision);
 bool dev_query(DeviceSolve* ds);

@@ -1984,6 +1984,31 @@ __global__ __launch_bounds__(64) void k_init_state_lds(const SolveDesc d, const 
 }
 
 static void set_lds_limit(const void* fn, size_t lds);
+
+// --------------------------------------------------------------------------
+// Allocation-free runs (dev_run_static): the current partials of the
+// propagation kernel live in this file-scope device array instead of a
+// per-solve buffer. Each GPU holds its own copy for the life of the process;
+// its address on a device comes from hipGetSymbolAddress, taken once per
+// device when the first solve is created there (host code never passes the
+// array's name, which is only a host-side stand-in). Runs on one GPU share it:
+// dev_run_static serialises them with g_static_mu.
+constexpr size_t STATIC_SCRATCH_DOUBLES = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[STATIC_SCRATCH_DOUBLES];
+constexpr int STATIC_MAX_DEVICES = 64;
+static std::mutex g_static_mu;
+static double* g_scratch_ptr[STATIC_MAX_DEVICES] = {};
+
+static void static_scratch_setup(int dev) {
+    if (dev < 0 || dev >= STATIC_MAX_DEVICES) throw std::runtime_error("device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_scratch_ptr[dev]) return;
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the static scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+}
+
 // The device initial state of 3..INIT_MAXNB bands keeps its matrices in LDS;
 // 1-2 bands use the thread-per-k kernel over 3 element-major scratch planes.
 static bool init_needs_scratch(int nb) { return !(nb >= 3 && nb <= INIT_MAXNB); }
@@ -1998,6 +2023,7 @@ struct DeviceSolve {
     double* hJ = nullptr;  // pinned host copy of J for asynchronous completion
     int last_C = 6;
     size_t jpart_bytes = 0;
+    int device = 0;         // HIP device current at creation (its stream's device)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk = nullptr;
     bool use_herm = true;   // paired Hermitian kernel when the blocks allow it
@@ -2033,6 +2059,8 @@ DeviceSolve* dev_create(const HostSolve& hs) {
     try {
         ds->d = hs.d;
         ds->dt = hs.dt;
+        HIP_CHECK(hipGetDevice(&ds->device));
+        static_scratch_setup(ds->device);
         HIP_CHECK(hipStreamCreateWithFlags(&ds->stream, hipStreamNonBlocking));
         HIP_CHECK(hipEventCreate(&ds->ev0));
         HIP_CHECK(hipEventCreate(&ds->ev1));
@@ -2420,14 +2448,18 @@ static void set_lds_limit(const void* fn, size_t lds) {
 
 // Launch the propagation kernel of one solve; returns the number of current
 // components per wave partial.
-static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, int& nwaves) {
+// jpart: when given, the current partials go there (the caller checked its
+// size) instead of the solve's own buffer.
+static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, int& nwaves,
+                       double* jpart = nullptr) {
     if (k.kind == KIND_THREAD && d.ns > k.b) throw std::runtime_error("thread-per-k kernel: support exceeds MAXU");
     int wpt = 0, block = 0;
     const int ntiles = tiles_of(d, k, wpt, block);
     nwaves = ntiles * wpt;
     const int C = comps_of(k);
-    ensure_jpart(ds, nwaves, C);
-    const InstPtrs ip = ptrs_of(ds, nwaves, sep_kpw(d, k));
+    if (!jpart) ensure_jpart(ds, nwaves, C);
+    InstPtrs ip = ptrs_of(ds, nwaves, sep_kpw(d, k));
+    if (jpart) ip.jpart = jpart;
     const size_t lds = lds_of(d, k);
     dispatch(k, [&](auto single, auto) {
         set_lds_limit((const void*)single, lds);
@@ -2570,6 +2602,40 @@ double dev_complete(DeviceSolve* ds, double* J_out) {
 double dev_run(DeviceSolve* ds, int precision, double* J_out) {
     dev_launch(ds, precision);
     return dev_complete(ds, J_out);
+}
+
+size_t static_scratch_bytes() { return sizeof(double) * STATIC_SCRATCH_DOUBLES; }
+
+// dev_run without allocating: the launch key of the default rule (or the
+// tuned one; no tuning here), the current partials in the static scratch of
+// the solve's device, the reduced current in the solve's own J buffer, copied
+// through its pinned host buffer into J_out. Refuses launches whose partials
+// exceed the scratch. Serialised process-wide: the scratch is shared.
+double dev_run_static(DeviceSolve* ds, int precision, double* J_out) {
+    const SolveDesc& d = ds->d;
+    const LaunchKey k = classify(ds, precision);
+    int wpt = 0, block = 0;
+    const size_t parts = (size_t)tiles_of(d, k, wpt, block) * wpt;
+    const size_t need = (size_t)comps_of(k) * d.nt * parts;
+    if (need > STATIC_SCRATCH_DOUBLES)
+        throw std::runtime_error("allocation-free run: the current partials need " + std::to_string(need * 8) +
+                                 " bytes (" + std::to_string(d.nt) + " steps x " + std::to_string(parts) +
+                                 " partials x " + std::to_string(comps_of(k)) +
+                                 " components), above the static scratch of " +
+                                 std::to_string(static_scratch_bytes()) + " bytes");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    double* ptr = g_scratch_ptr[ds->device];
+    if (!ptr) throw std::runtime_error("allocation-free run: static scratch not set up on this device");
+    int nwaves = 0;
+    HIP_CHECK(hipEventRecord(ds->ev0, ds->stream));
+    const int C = launch_with(ds, k, d, nwaves, ptr);
+    HIP_CHECK(hipEventRecord(ds->evk, ds->stream));
+    const OMap om = omap_of(ds, k);
+    hipLaunchKernelGGL(k_reduce_j, dim3(d.nt), dim3(64), 0, ds->stream, ptr, nwaves, C, om, (double)d.N, ds->J);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(ds->hJ, ds->J, sizeof(double) * 6 * (size_t)d.nt, hipMemcpyDeviceToHost, ds->stream));
+    HIP_CHECK(hipEventRecord(ds->ev1, ds->stream));
+    return dev_complete(ds, J_out);  // waits for the stream: the scratch is free again on return
 }
 
 

@@ -150,6 +150,21 @@ API void SWESim_run_simulation(SWESim* s) {
     });
 }
 
+// SWESim_run_simulation without device allocation: J(t) of this run goes to
+// the caller's J_out ([nt][6] doubles: re/im of x, y, z) instead of the
+// handle's current record, the current partials to the library's static
+// scratch. Host-only state is refused like run_simulation; the FFT form,
+// which allocates its own fields, is refused.
+API int SWESim_run_simulation_static(SWESim* s, double* J_out) {
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_run_simulation_static: call SWESim_init first");
+        if (!J_out) throw std::runtime_error("SWESim_run_simulation_static: null output");
+        if (s->variant == 2) throw std::runtime_error("SWESim_run_simulation_static: not available for the FFT form");
+        s->last_ms = hhg::dev_run_static(s->dev, s->precision, J_out);
+    });
+    return s ? s->status : -1;
+}
+
 // Text dumps of src/swe.cpp:131-181 into test/ and fields/ under the working
 // directory (both created when missing); the solve's state is not changed.
 API void SWESim_test_files(SWESim* s) {

@@ -146,6 +146,22 @@ double SWESim_last_run_ms(SWESim* swe);
 /* Device time of the propagation kernel alone in the last run (ms). */
 double SWESim_last_kernel_ms(SWESim* swe);
 
+/* ---- allocation-free run (no counterpart in the reference) --------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ * SWESim_run_simulation_static propagates nt steps from the current state
+ * like SWESim_run_simulation and writes this run's J(t) to the caller's
+ * J_out, nt x 6 doubles (re/im of x, y, z per step), instead of appending it
+ * to the handle's record. It calls no hipMalloc/hipFree: the current
+ * partials of the propagation kernel (steps x partials x components doubles)
+ * live in the library's static device array of 64 MiB per GPU (held until
+ * the process ends); a solve whose partials exceed it is refused with an
+ * error (e.g. the 2-band 200 x 200 grid at 4096 steps needs 61.4 MB, a
+ * 240 x 240 one 88 MB). Calls of one process on one GPU share that array and
+ * are serialised by the library. Not available for the FFT form (kernel
+ * variant 2). Returns 0, or -1/1 with the handle's error set. */
+int SWESim_run_simulation_static(SWESim* swe, double* J_out);
+
 /* ---- batches (no counterpart in the reference) -------------------------
  * A batch propagates many initialised solves together: solves that use the
  * same kernel share one launch whose workgroups interleave their k-tiles;
