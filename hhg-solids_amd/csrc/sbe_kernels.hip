@@ -1256,7 +1256,10 @@ __device__ __forceinline__ double wave_total(double v) {
     return (rl(0) + rl(16)) + (rl(32) + rl(48));
 }
 
-template <int NBP, int MAXD, typename T>
+// WIDE: the 8-band FP32 form for workgroups of 9-12 waves (3 per SIMD within
+// 168 VGPRs): no LDS staging of the blocks (the LDS holds the wider
+// workgroup's tiles instead) and the field row through scalar loads.
+template <int NBP, int MAXD, typename T, bool WIDE = false>
 __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip, int tile) {
     constexpr int HB = NBP / 2, LPK = HB * HB, GPW = 64 / LPK, E = NBP * NBP;
     using TT = Tile<T, NBP>;
@@ -1264,7 +1267,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     extern __shared__ __align__(16) unsigned char smem[];
     const int W = blockDim.x >> 6;
     // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
-    constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8;
+    constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8 && !WIDE;
     const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd);
     const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
@@ -1338,8 +1341,14 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     auto build = [&](int ti, int buf) {
         const double* st = ip.steptab + (size_t)ti * d.stride;
         double ef[9];
+        if constexpr (WIDE) {  // wave-uniform values in SGPRs
+            const cdptr sts = as_const(st);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) ef[c] = DB ? efs[(ti & 1) * 16 + c] : st[c];
+            for (int c = 0; c < 9; ++c) ef[c] = sts[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) ef[c] = DB ? efs[(ti & 1) * 16 + c] : st[c];
+        }
         const cx<double>* wq = wv + (ti & 1) * MAX_SUPPORT;
         cx<T>* G = reinterpret_cast<cx<T>*>(smem + L.off_g + buf * L.gbytes);
         cx<double>* GV = reinterpret_cast<cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
@@ -1367,6 +1376,16 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
                         for (int o = OP_V; o < NOPS; ++o) ob[o] = {0.0, 0.0};
                     }
+                } else if constexpr (WIDE) {
+                    // the H and rbc blocks rounded to FP32 as the staged form
+                    // holds them, so both forms build identical tables
+#pragma unroll
+                    for (int o = 0; o < OP_V; ++o) {
+                        const cx<double> b = ld<double>(bs + 2 * (size_t)o * nbe);
+                        ob[o] = {(double)(float)b.re, (double)(float)b.im};
+                    }
+#pragma unroll
+                    for (int o = OP_V; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
                 } else {
 #pragma unroll
                     for (int o = 0; o < NOPS; ++o) ob[o] = ld<double>(bs + 2 * (size_t)o * nbe);
@@ -1455,14 +1474,14 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         // this k-point's twiddles for the three stage sums: registers for NBP <= 8,
         // LDS broadcasts for NBP = 16 (whose FP64 kernel has no registers left)
         constexpr bool ZREG = KPW_MAX > 1 && NBP != 16;
-        cx<double> zr[ZREG ? MAXD : 1];
+        cx<T> zr[ZREG ? MAXD : 1];  // in the stage type: they only feed the stage sums
         if constexpr (ZREG) {
 #pragma unroll
-            for (int dd = 0; dd < MAXD; ++dd) zr[dd] = zval(p, dd);
+            for (int dd = 0; dd < MAXD; ++dd) zr[dd] = cast<T>(zval(p, dd));
         }
-        auto zk = [&](int dd) -> cx<double> {
+        auto zk = [&](int dd) -> cx<T> {
             if constexpr (ZREG) return zr[dd];
-            else return zval(p, dd);
+            else return cast<T>(zval(p, dd));
         };
         cx<T> Hs[4], rs[4], K[4], acc[4];
         auto hstage = [&](int j) {
@@ -1470,7 +1489,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             for (int q = 0; q < 4; ++q) Hs[q] = {(T)0, (T)0};
 #pragma unroll
             for (int dd = 0; dd < MAXD; ++dd) {
-                const cx<T> zz = cast<T>(zk(dd));
+                const cx<T> zz = zk(dd);
                 const cx<T>* gp = G + (size_t)(j * MAXD + dd) * E;
                 cx<T> g0, g1, g2, g3;
                 TT::ld2(gp, i0, j0, g0, g1);
@@ -1701,6 +1720,17 @@ template <int NBP, int MAXD, typename T>
 __global__ __launch_bounds__(512) void k_sep_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
     const int2 t = tiles[blockIdx.x];
     sep_body<NBP, MAXD, T>(insts[t.x].d, insts[t.x].p, t.y);
+}
+constexpr int SEP_WIDE_MAX = 12;  // waves per workgroup of the wide 8-band FP32 form
+template <int MAXD>
+__global__ __launch_bounds__(64 * SEP_WIDE_MAX) void k_sep_wide(const SolveDesc d, const InstPtrs ip) {
+    sep_body<8, MAXD, float, true>(d, ip, blockIdx.x);
+}
+template <int MAXD>
+__global__ __launch_bounds__(64 * SEP_WIDE_MAX) void k_sep_wide_batch(const DevInst* __restrict__ insts,
+                                                                       const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    sep_body<8, MAXD, float, true>(insts[t.x].d, insts[t.x].p, t.y);
 }
 
 template <int EPL, typename T>
@@ -2221,20 +2251,25 @@ static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
 }
 // w <= 0: waves per workgroup by the default rule (equal chunks of a k-row,
 // at most 8 waves, reduced until the LDS footprint fits)
-static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w) {
+// wide: the 8-band FP32 form with up to SEP_WIDE_MAX waves (k_sep_wide; no
+// LDS staging), valid only for FP32 solves of 5-8 bands.
+static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool wide = false) {
     const SolveDesc& d = ds->d;
     LaunchKey k;
     k.fp32 = fp32;
     if (!(d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep)) return k;
     const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
+    if (wide && !(fp32 && nbp == 8)) return k;
     const int maxd = d.nd <= 5 ? 5 : 8;
-    const size_t nstage = (fp32 && nbp <= 8) ? (size_t)d.ns * d.nb * d.nb : 0;
+    const size_t nstage = (fp32 && nbp <= 8 && !wide) ? (size_t)d.ns * d.nb * d.nb : 0;
+    const int wmax = wide ? SEP_WIDE_MAX : 8;
     if (w <= 0) {
-        const int chunks = (d.N2 + 8 * gpw - 1) / (8 * gpw);
+        const int chunks = (d.N2 + wmax * gpw - 1) / (wmax * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
         while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) --w;
     }
-    if (w < 1 || w > 8 || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) return k;
+    if (w < 1 || w > wmax || (wide && w <= 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX)
+        return k;
     k.kind = KIND_SEP;
     k.a = nbp;
     k.b = w;
@@ -2242,7 +2277,11 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w) {
     return k;
 }
 
-static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
+// batch: the solve runs in a batch, where solves of up to 8 bands keep one
+// k-point per lane group; its 5-8-band FP32 workgroups then widen to the
+// wide form (up to 12 waves: 3 per SIMD) when the row allows more than 8.
+static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch = false);
+static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool batch = false) {
     const SolveDesc& d = ds->d;
     const int fp32 = precision != 0;
     LaunchKey k;
@@ -2251,7 +2290,14 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
     if ((k = key_pair(ds, fp32)).kind) return k;
     if ((k = key_thread(ds, fp32)).kind) return k;
     if (d.nb > 32) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 32)");
-    if ((k = key_sep(ds, fp32, 0)).kind) return k;
+    if (batch && (k = key_sep(ds, fp32, 0, true)).kind) return k;
+    if ((k = key_sep(ds, fp32, 0)).kind) {
+        // a single solve whose lane groups carry one k-point each takes the
+        // wide form when it applies (8 bands, 200 x 200: 191 -> 150 ms)
+        LaunchKey kw;
+        if (sep_kpw(d, k) == 1 && (kw = key_sep(ds, fp32, 0, true)).kind) return kw;
+        return k;
+    }
     k.kind = KIND_ELEM;
     k.fp32 = fp32;
     // elements per lane (lanes_per_k lanes per k-point: <= 64 up to 16 bands, 256 beyond)
@@ -2261,14 +2307,14 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision) {
 
 // The key a run uses: the autotuned one when the solve has been tuned for
 // this precision, else the default rule.
-static LaunchKey classify(const DeviceSolve* ds, int precision) {
+static LaunchKey classify(const DeviceSolve* ds, int precision, bool batch = false) {
     const int p = precision != 0;
     if (ds->tuned[p]) {
         LaunchKey k;
         k.kind = ds->tkey[p][0]; k.a = ds->tkey[p][1]; k.b = ds->tkey[p][2]; k.c = ds->tkey[p][3]; k.fp32 = p;
         return k;
     }
-    return classify_default(ds, precision);
+    return classify_default(ds, precision, batch);
 }
 
 static int comps_of(const LaunchKey& k) {
@@ -2312,7 +2358,7 @@ static int device_cus() {
 // instances keep every CU busy, so the 8-band gain from fewer lifetimes is
 // absent while the longer ones lengthen the batch's tail (measured 4 % slower
 // on a 64-instance mix): batched solves of up to 8 bands keep one k-point.
-static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch = false) {
+static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch) {
     if (k.kind != KIND_SEP || sep_kpw_max(k.a) == 1 || (batch && k.a <= 8)) return 1;
     const int cus = device_cus();
     const int kpt1 = k.b * (64 / ((k.a / 2) * (k.a / 2)));
@@ -2349,7 +2395,8 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
     if (k.kind == KIND_SEP)
-        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd)
+        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8 && k.b <= 8) ? (size_t)d.ns * d.nb * d.nb : 0,
+                          d.nwd)
             .bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = lanes_per_k(d.nb);
@@ -2399,6 +2446,12 @@ static void dispatch(const LaunchKey& k, F&& f) {
 #undef THR
             break;
         case KIND_SEP:
+            if (k.a == 8 && k.b > 8) {  // the wide 8-band form, FP32 only
+                if (!k.fp32) break;
+                if (k.c == 5) return f(k_sep_wide<5>, k_sep_wide_batch<5>);
+                if (k.c == 8) return f(k_sep_wide<8>, k_sep_wide_batch<8>);
+                break;
+            }
 #define SEP(NBP, D)                                                                                     \
     if (k.a == NBP && k.c == D)                                                                          \
         return go(k_sep<NBP, D, double>, k_sep_batch<NBP, D, double>, k_sep<NBP, D, float>,              \
@@ -2508,8 +2561,10 @@ static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
             if (!(o < k) && !(k < o)) return;
         out.push_back(k);
     };
-    if (out[0].kind == KIND_SEP)
+    if (out[0].kind == KIND_SEP) {
         for (int w = 1; w <= 8; ++w) add(key_sep(ds, fp32, w));
+        for (int w = 9; w <= SEP_WIDE_MAX; ++w) add(key_sep(ds, fp32, w, true));
+    }
     if (ds->pd.nrep > 0 && ds->use_herm) {
         add(key_bloch(ds, fp32));
         add(key_pair(ds, fp32));
@@ -2686,7 +2741,7 @@ double batch_run(Batch* b, const std::vector<DeviceSolve*>& items, const std::ve
     for (size_t i = 0; i < items.size(); ++i)
         if (g_autotune && !items[i]->tuned[prec[i] != 0]) dev_autotune(items[i], prec[i], 32);
     for (size_t i = 0; i < items.size(); ++i) {
-        keys[i] = classify(items[i], prec[i]);
+        keys[i] = classify(items[i], prec[i], true);
         groups[keys[i]].push_back((int)i);
     }
     while (b->streams.size() < groups.size()) {

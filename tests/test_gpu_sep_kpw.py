@@ -82,3 +82,32 @@ def test_kpoints_per_wave_in_a_batch(tmp_path):
         assert s.kpoints_per_wave() in (2, 4)
         assert rel_l2(Jb, Ja) <= SAME_TOL
         s.close()
+
+
+def test_wide_workgroups_of_the_8_band_fp32_kernel(tmp_path, oracle_lib):
+    """FP32 solves of 5-8 bands whose lane groups carry one k-point take
+    workgroups of 9-12 waves (3 per SIMD; the blocks are rounded to FP32 as
+    the staged form holds them): rows of 40 k-points in one 10-wave workgroup.
+    Against the oracle, and batched against alone (a batched 8-band member
+    takes the wide form too)."""
+    from hhgamd import lib
+    params = [_param(_model(tmp_path, nb), (24, 40), "fp32") for nb in (8, 6)]
+    alone = []
+    for p in params:
+        with lib.Solve(p) as s:
+            L, kpw = s.launch(), s.kpoints_per_wave()
+            s.run()
+            J = s.current()[1]
+        _, Jref = oracle_lib.run(p["path_tb"], p, mode="perk")
+        err = rel_l2(J, Jref)
+        print(f"nb={s.nb} 24x40 fp32: waves per workgroup {L['b']}, {kpw} k-point per lane group, "
+              f"rel L2 vs oracle {err:.2e}")
+        assert L["kernel"] == 6 and L["a"] == 8 and L["b"] == 10 and kpw == 1
+        assert err <= FP32_TOL
+        alone.append(J)
+    solves = [lib.Solve(p) for p in params]
+    with lib.Batch(solves) as b:
+        b.run()
+    for s, Ja in zip(solves, alone):
+        assert rel_l2(s.current()[1], Ja) <= SAME_TOL
+        s.close()
