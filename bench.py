@@ -35,7 +35,7 @@ UNIT = "k-point·RK4-steps/s"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X published vector peaks
 ATTAINABLE_FP32 = 78.6  # non-packed FP32 VALU: one FMA per lane and instruction, the FP64 rate
 PEAK_HBM_GBS = 8000.0
-KERNELS = {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane", 4: "k_pair2s", 6: "k_sep", 8: "k_bloch2"}
+KERNELS = {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane", 4: "k_pair2s", 6: "k_sep", 8: "k_bloch2", 9: "k_sep_lean"}
 
 
 def j_errors(t, J, Jref, lam):
@@ -476,7 +476,7 @@ def bench_batch(args):
     kinds = {}
     for s, p in zip(solves, params):
         L = s.launch()
-        key = f"nb{s.nb}/k{L['kernel']}" + (f"/w{L['b']}" if L["kernel"] == 6 else "")
+        key = f"nb{s.nb}/k{L['kernel']}" + (f"/w{L['b']}" if L["kernel"] in (6, 9) else "")
         kinds[key] = kinds.get(key, 0) + 1
     # plain FP64 path, one solve at a time, same instances; J(t) of each for the error check
     errs, worsts = [], []

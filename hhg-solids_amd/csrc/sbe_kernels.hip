@@ -1256,10 +1256,11 @@ __device__ __forceinline__ double wave_total(double v) {
     return (rl(0) + rl(16)) + (rl(32) + rl(48));
 }
 
-// WIDE: the 8-band FP32 form for workgroups of 9-12 waves (3 per SIMD within
-// 168 VGPRs): no LDS staging of the blocks (the LDS holds the wider
-// workgroup's tiles instead) and the field row through scalar loads.
-template <int NBP, int MAXD, typename T, bool WIDE = false>
+// LEAN: the FP32 form of 5-8 bands within 168 VGPRs (3 waves per SIMD):
+// no LDS staging of the blocks (the H and rbc blocks are rounded to FP32 as
+// they are read, so the tables equal the staged form's) and the field row
+// through scalar loads.
+template <int NBP, int MAXD, typename T, bool LEAN = false>
 __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip, int tile) {
     constexpr int HB = NBP / 2, LPK = HB * HB, GPW = 64 / LPK, E = NBP * NBP;
     using TT = Tile<T, NBP>;
@@ -1267,7 +1268,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     extern __shared__ __align__(16) unsigned char smem[];
     const int W = blockDim.x >> 6;
     // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
-    constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8 && !WIDE;
+    constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8 && !LEAN;
     const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd);
     const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
@@ -1341,7 +1342,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     auto build = [&](int ti, int buf) {
         const double* st = ip.steptab + (size_t)ti * d.stride;
         double ef[9];
-        if constexpr (WIDE) {  // wave-uniform values in SGPRs
+        if constexpr (LEAN) {  // wave-uniform values in SGPRs
             const cdptr sts = as_const(st);
 #pragma unroll
             for (int c = 0; c < 9; ++c) ef[c] = sts[c];
@@ -1376,7 +1377,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
                         for (int o = OP_V; o < NOPS; ++o) ob[o] = {0.0, 0.0};
                     }
-                } else if constexpr (WIDE) {
+                } else if constexpr (LEAN) {
                     // the H and rbc blocks rounded to FP32 as the staged form
                     // holds them, so both forms build identical tables
 #pragma unroll
@@ -1721,16 +1722,18 @@ __global__ __launch_bounds__(512) void k_sep_batch(const DevInst* __restrict__ i
     const int2 t = tiles[blockIdx.x];
     sep_body<NBP, MAXD, T>(insts[t.x].d, insts[t.x].p, t.y);
 }
-constexpr int SEP_WIDE_MAX = 12;  // waves per workgroup of the wide 8-band FP32 form
-template <int MAXD>
-__global__ __launch_bounds__(64 * SEP_WIDE_MAX) void k_sep_wide(const SolveDesc d, const InstPtrs ip) {
-    sep_body<8, MAXD, float, true>(d, ip, blockIdx.x);
+// The lean FP32 form of 5-8 bands: at most 12 waves per workgroup, so at
+// most 168 VGPRs (3 waves per SIMD) instead of the staged form's 185.
+constexpr int SEP_LEAN_MAX = 12;
+template <int NBP, int MAXD>
+__global__ __launch_bounds__(64 * SEP_LEAN_MAX) void k_sep_lean(const SolveDesc d, const InstPtrs ip) {
+    sep_body<NBP, MAXD, float, true>(d, ip, blockIdx.x);
 }
-template <int MAXD>
-__global__ __launch_bounds__(64 * SEP_WIDE_MAX) void k_sep_wide_batch(const DevInst* __restrict__ insts,
+template <int NBP, int MAXD>
+__global__ __launch_bounds__(64 * SEP_LEAN_MAX) void k_sep_lean_batch(const DevInst* __restrict__ insts,
                                                                        const int2* __restrict__ tiles) {
     const int2 t = tiles[blockIdx.x];
-    sep_body<8, MAXD, float, true>(insts[t.x].d, insts[t.x].p, t.y);
+    sep_body<NBP, MAXD, float, true>(insts[t.x].d, insts[t.x].p, t.y);
 }
 
 template <int EPL, typename T>
@@ -2197,7 +2200,7 @@ size_t dev_bytes(const HostSolve& hs) {
 }
 
 // Which kernel and template instance a solve runs with.
-enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8 };
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8, KIND_SEPL = 9 };
 constexpr size_t LDS_MAX = 160 * 1024;
 struct LaunchKey {
     int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
@@ -2251,26 +2254,28 @@ static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
 }
 // w <= 0: waves per workgroup by the default rule (equal chunks of a k-row,
 // at most 8 waves, reduced until the LDS footprint fits)
-// wide: the 8-band FP32 form with up to SEP_WIDE_MAX waves (k_sep_wide; no
-// LDS staging), valid only for FP32 solves of 5-8 bands.
-static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool wide = false) {
+// lean: the lean FP32 form (k_sep_lean, KIND_SEPL; no LDS staging, at most
+// 168 VGPRs) of 5-8 bands with up to 12 waves per workgroup (one workgroup,
+// 3 waves per SIMD, per CU). For 3-4 bands it measured slower (120 x 120:
+// 23.1 ms staged, 29.4 ms lean with 4 waves), so they keep the staged form.
+static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = false) {
     const SolveDesc& d = ds->d;
     LaunchKey k;
     k.fp32 = fp32;
     if (!(d.nb >= 3 && d.nb <= 16 && d.hermhx && d.nd > 0 && ds->use_sep)) return k;
     const int nbp = d.nb <= 4 ? 4 : d.nb <= 8 ? 8 : 16, gpw = 64 / ((nbp / 2) * (nbp / 2));
-    if (wide && !(fp32 && nbp == 8)) return k;
+    if (lean && !(fp32 && nbp == 8)) return k;
     const int maxd = d.nd <= 5 ? 5 : 8;
-    const size_t nstage = (fp32 && nbp <= 8 && !wide) ? (size_t)d.ns * d.nb * d.nb : 0;
-    const int wmax = wide ? SEP_WIDE_MAX : 8;
+    const size_t nstage = (fp32 && nbp <= 8 && !lean) ? (size_t)d.ns * d.nb * d.nb : 0;
+    const int wmax = lean ? SEP_LEAN_MAX : 8;
     if (w <= 0) {
         const int chunks = (d.N2 + wmax * gpw - 1) / (wmax * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
         while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) --w;
     }
-    if (w < 1 || w > wmax || (wide && w <= 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX)
+    if (w < 1 || w > (lean ? SEP_LEAN_MAX : 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX)
         return k;
-    k.kind = KIND_SEP;
+    k.kind = lean ? KIND_SEPL : KIND_SEP;
     k.a = nbp;
     k.b = w;
     k.c = maxd;
@@ -2278,8 +2283,7 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool wide = fal
 }
 
 // batch: the solve runs in a batch, where solves of up to 8 bands keep one
-// k-point per lane group; its 5-8-band FP32 workgroups then widen to the
-// wide form (up to 12 waves: 3 per SIMD) when the row allows more than 8.
+// k-point per lane group; 5-8-band FP32 members then take the lean form.
 static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch = false);
 static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool batch = false) {
     const SolveDesc& d = ds->d;
@@ -2293,7 +2297,7 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
     if (batch && (k = key_sep(ds, fp32, 0, true)).kind) return k;
     if ((k = key_sep(ds, fp32, 0)).kind) {
         // a single solve whose lane groups carry one k-point each takes the
-        // wide form when it applies (8 bands, 200 x 200: 191 -> 150 ms)
+        // lean form when it applies (8 bands, 200 x 200: 191 -> 150 ms)
         LaunchKey kw;
         if (sep_kpw(d, k) == 1 && (kw = key_sep(ds, fp32, 0, true)).kind) return kw;
         return k;
@@ -2375,7 +2379,7 @@ static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch) {
 
 // Tiles (workgroups) of one instance and waves per tile.
 static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile, int& block, bool batch = false) {
-    if (k.kind == KIND_SEP) {
+    if (k.kind == KIND_SEP || k.kind == KIND_SEPL) {
         block = 64 * k.b;
         waves_per_tile = 1;  // one current partial per workgroup and step
         const int kpt = k.b * (64 / ((k.a / 2) * (k.a / 2))) * sep_kpw(d, k, batch);
@@ -2395,9 +2399,9 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
     if (k.kind == KIND_SEP)
-        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8 && k.b <= 8) ? (size_t)d.ns * d.nb * d.nb : 0,
-                          d.nwd)
+        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd)
             .bytes;
+    if (k.kind == KIND_SEPL) return sep_layout(k.a, k.c, 4, k.b, 0, d.nwd).bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = lanes_per_k(d.nb);
     const int kpb = 256 / G;
@@ -2446,18 +2450,17 @@ static void dispatch(const LaunchKey& k, F&& f) {
 #undef THR
             break;
         case KIND_SEP:
-            if (k.a == 8 && k.b > 8) {  // the wide 8-band form, FP32 only
-                if (!k.fp32) break;
-                if (k.c == 5) return f(k_sep_wide<5>, k_sep_wide_batch<5>);
-                if (k.c == 8) return f(k_sep_wide<8>, k_sep_wide_batch<8>);
-                break;
-            }
 #define SEP(NBP, D)                                                                                     \
     if (k.a == NBP && k.c == D)                                                                          \
         return go(k_sep<NBP, D, double>, k_sep_batch<NBP, D, double>, k_sep<NBP, D, float>,              \
                   k_sep_batch<NBP, D, float>);
             SEP(4, 5) SEP(4, 8) SEP(8, 5) SEP(8, 8) SEP(16, 5) SEP(16, 8)
 #undef SEP
+            break;
+        case KIND_SEPL:  // FP32 only
+            if (!k.fp32) break;
+            if (k.a == 8 && k.c == 5) return f(k_sep_lean<8, 5>, k_sep_lean_batch<8, 5>);
+            if (k.a == 8 && k.c == 8) return f(k_sep_lean<8, 8>, k_sep_lean_batch<8, 8>);
             break;
         case KIND_ELEM:
             if (k.a == 1)
@@ -2561,9 +2564,9 @@ static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
             if (!(o < k) && !(k < o)) return;
         out.push_back(k);
     };
-    if (out[0].kind == KIND_SEP) {
+    if (out[0].kind == KIND_SEP || out[0].kind == KIND_SEPL) {
         for (int w = 1; w <= 8; ++w) add(key_sep(ds, fp32, w));
-        for (int w = 9; w <= SEP_WIDE_MAX; ++w) add(key_sep(ds, fp32, w, true));
+        for (int w = 1; w <= SEP_LEAN_MAX; ++w) add(key_sep(ds, fp32, w, true));
     }
     if (ds->pd.nrep > 0 && ds->use_herm) {
         add(key_bloch(ds, fp32));
@@ -2888,7 +2891,7 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     }
     int nv = 0, nx = 0;
     for (int a = 0; a < 3; ++a) { nv += (d.vmask >> a) & 1; nx += (d.xmask >> a) & 1; }
-    if (kind == KIND_SEP) {
+    if (kind == KIND_SEP || kind == KIND_SEPL) {
         // padded size P, table depth D (MAXD): stage sums 3 D P^2 cMAC,
         // products 4 P^3 cMAC, the M - M^dagger combination and RK4 axpys
         // ~26 P^2 flops, current 3 D (4 + 1) cMAC per 2x2 lane block; plus the
