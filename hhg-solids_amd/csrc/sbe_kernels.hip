@@ -901,21 +901,26 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
 }
 
 // --------------------------------------------------------------------------
-// Element-per-lane kernel for any nb <= 32. A group of G lanes owns one k
+// Element-per-lane kernel for any nb <= 64. A group of G lanes owns one k
 // (G = nb*nb rounded up to a power of two, at most 64, for nb <= 16; the whole
-// 256-lane workgroup for 17 <= nb <= 32); lane l owns the matrix elements
-// e = l + G*q, q < EPL (1 for nb <= 8, 4 for 9-16, 2 while nb*nb <= 512, else
-// 4). Per stage the stage Hamiltonian and stage rho are staged through LDS and
-// each lane forms its commutator elements from row i / column j there. LDS per
-// k: the support phases (ns complex, FP64) and two nb*nb matrices of type T.
+// 256-lane workgroup for 17 <= nb <= 32, a 1024-lane one for 33-64); lane l
+// owns the matrix elements e = l + G*q, q < EPL (1 for nb <= 8, 4 for 9-16,
+// 2 while nb*nb <= 512, else 4). Per stage the stage Hamiltonian and stage rho
+// are staged through LDS and each lane forms its commutator elements from
+// row i / column j there. LDS per k: the support phases (ns complex, FP64) and
+// two nb*nb matrices of type T (128 KiB at 64 bands in FP64). BIG (33-64
+// bands, 128 VGPRs at 16 waves per workgroup): the stage Hamiltonians are
+// summed afresh for each stage field instead of keeping the field-free sums of
+// H and rbc in registers.
 __host__ __device__ inline int lanes_per_k(int nb) {
+    if (nb > 32) return 1024;
     if (nb > 16) return 256;
     int g = 1;
     while (g < nb * nb && g < 64) g <<= 1;
     return g;
 }
 
-template <int EPL, typename T>
+template <int EPL, typename T, bool BIG = false>
 __device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip, int bid) {
     const double* __restrict__ blk = ip.blk;
     const double* __restrict__ steptab = ip.steptab;
@@ -969,38 +974,66 @@ __device__ __forceinline__ void elem_body(const SolveDesc& d, const InstPtrs& ip
             wS[s] = cmul(cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2)), ld<double>(st + STEP_HEAD + 2 * s));
         }
         __syncthreads();
-        cx<T> Ht[EPL], Xt[3][EPL];
+        constexpr int KEEP = BIG ? 1 : EPL;  // field-free sums kept in registers (not for BIG)
+        cx<T> Ht[KEEP], Xt[3][KEEP];
+        if constexpr (!BIG) {
 #pragma unroll
-        for (int q = 0; q < EPL; ++q) {
-            Ht[q] = {0, 0};
-            Xt[0][q] = Xt[1][q] = Xt[2][q] = {0, 0};
-        }
-        for (int s = 0; s < d.ns; ++s) {
-            const uint32_t om = d.opmask[s];
-            const cx<T> w = cast<T>(wS[s]);
-            const double* b = blk + (size_t)s * NOPS * E * 2;
-            if (om & (1u << OP_H)) {
-#pragma unroll
-                for (int q = 0; q < EPL; ++q) cmac(Ht[q], w, ld<T>(b + 2 * (OP_H * E + ei[q])));
+            for (int q = 0; q < EPL; ++q) {
+                Ht[q] = {0, 0};
+                Xt[0][q] = Xt[1][q] = Xt[2][q] = {0, 0};
             }
-            for (int a = 0; a < 3; ++a)
-                if (om & (1u << (OP_X + a))) {
+            for (int s = 0; s < d.ns; ++s) {
+                const uint32_t om = d.opmask[s];
+                const cx<T> w = cast<T>(wS[s]);
+                const double* b = blk + (size_t)s * NOPS * E * 2;
+                if (om & (1u << OP_H)) {
 #pragma unroll
-                    for (int q = 0; q < EPL; ++q) cmac(Xt[a][q], w, ld<T>(b + 2 * ((OP_X + a) * E + ei[q])));
+                    for (int q = 0; q < EPL; ++q) cmac(Ht[q], w, ld<T>(b + 2 * (OP_H * E + ei[q])));
                 }
+                for (int a = 0; a < 3; ++a)
+                    if (om & (1u << (OP_X + a))) {
+#pragma unroll
+                        for (int q = 0; q < EPL; ++q) cmac(Xt[a][q], w, ld<T>(b + 2 * ((OP_X + a) * E + ei[q])));
+                    }
+            }
         }
         cx<T> Hs[EPL], rs[EPL], K[EPL], acc[EPL];
         auto stage_h = [&](int js) {
-#pragma unroll
-            for (int q = 0; q < EPL; ++q) {
-                cx<T> h = Ht[q];
-                for (int a = 0; a < 3; ++a)
-                    if (d.xmask & (1 << a)) {
-                        const T f = (T)st[3 * js + a];
-                        h.re += f * Xt[a][q].re;
-                        h.im += f * Xt[a][q].im;
+            if constexpr (BIG) {
+                // sum_s w_s (H_s + E_js . rbc_s), in the same operation order as
+                // the kept sums below: H and each rbc axis summed over s first;
+                // one element at a time (register pressure)
+#pragma unroll 1
+                for (int q = 0; q < EPL; ++q) {
+                    cx<T> h = {0, 0}, x[3] = {{0, 0}, {0, 0}, {0, 0}};
+                    for (int s = 0; s < d.ns; ++s) {
+                        const uint32_t om = d.opmask[s];
+                        const cx<T> w = cast<T>(wS[s]);
+                        const double* b = blk + (size_t)s * NOPS * E * 2;
+                        if (om & (1u << OP_H)) cmac(h, w, ld<T>(b + 2 * (OP_H * E + ei[q])));
+                        for (int a = 0; a < 3; ++a)
+                            if (om & (1u << (OP_X + a))) cmac(x[a], w, ld<T>(b + 2 * ((OP_X + a) * E + ei[q])));
                     }
-                Hs[q] = h;
+                    for (int a = 0; a < 3; ++a)
+                        if (d.xmask & (1 << a)) {
+                            const T f = (T)st[3 * js + a];
+                            h.re += f * x[a].re;
+                            h.im += f * x[a].im;
+                        }
+                    Hs[q] = h;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < EPL; ++q) {
+                    cx<T> h = Ht[q];
+                    for (int a = 0; a < 3; ++a)
+                        if (d.xmask & (1 << a)) {
+                            const T f = (T)st[3 * js + a];
+                            h.re += f * Xt[a][q].re;
+                            h.im += f * Xt[a][q].im;
+                        }
+                    Hs[q] = h;
+                }
             }
         };
         auto deriv = [&](bool new_h) {  // K = -i (Hs rs - rs Hs), element-wise
@@ -1139,6 +1172,7 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // launch; only the velocity blocks of the window supports come from HBM.
 struct SepLayout {
     size_t off_g, gbytes, off_gv, gvbytes, off_sup, off_w, off_b, off_k, kbytes, off_z, off_jw, bytes;
+    int nsw;  // support slots of the per-support tables (ns rounded up to 4)
 };
 // Tables of NBP <= 8 are double-buffered (built one step ahead); NBP = 16
 // keeps one buffer and builds between two barriers.
@@ -1150,8 +1184,9 @@ __host__ __device__ constexpr int sep_kpw_max(int NBP) { return NBP == 16 ? 4 : 
 // (FP32 path, NBP <= 8: ns nb^2), 0 when the table build reads them from HBM.
 // nwd: values d with a velocity block; for NBP = 16, where the LDS bounds the
 // waves per workgroup, the velocity tables GV hold only those (by rank)
-__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage, int nwd) {
+__host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, int W, size_t nstage, int nwd, int ns) {
     SepLayout L;
+    L.nsw = (ns + 3) & ~3;
     const size_t E = (size_t)NBP * NBP;
     const int GPW = 64 / ((NBP / 2) * (NBP / 2));
     const int nbuf = sep_nbuf(NBP);
@@ -1160,8 +1195,8 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     L.off_g = o; o = align16(o + nbuf * L.gbytes);
     L.gvbytes = (size_t)(NBP == 16 ? nwd : MAXD) * 3 * E * 16;
     L.off_gv = o; o = align16(o + nbuf * L.gvbytes);
-    L.off_sup = o; o = align16(o + MAX_SUPPORT * 16 + MAX_SUPPORT * 4 + 16 * 4 + (size_t)MAXD * E * 2);
-    if (nbuf == 2) { L.off_w = o; o = align16(o + 2 * (MAX_SUPPORT * 16 + 16 * 8)); }  // weights + field rows
+    L.off_sup = o; o = align16(o + (size_t)L.nsw * 16 + (size_t)L.nsw * 4 + 16 * 4 + (size_t)MAXD * E * 2);
+    if (nbuf == 2) { L.off_w = o; o = align16(o + 2 * ((size_t)L.nsw * 16 + 16 * 8)); }  // weights + field rows
     else L.off_w = 0;
     L.off_b = o; o = align16(o + nstage * 4 * 8);
     L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
@@ -1269,7 +1304,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int W = blockDim.x >> 6;
     // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
     constexpr bool STAGE = std::is_same<T, float>::value && NBP <= 8 && !LEAN;
-    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd);
+    const SepLayout L = sep_layout(NBP, MAXD, (int)sizeof(T), W, STAGE ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd, d.ns);
     const int nb = d.nb, nbe = nb * nb, nd = d.nd, N = d.N, N1 = d.N1, N2 = d.N2;
     const int tid = threadIdx.x, bd = blockDim.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -1290,8 +1325,8 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     // per-support table in n2-sorted order: row twiddle tw1[m1 n1_s], index s;
     // segment bounds of each value d; entry codes (d << 16 | el << 8 | i << 4 | j)
     cx<double>* sw1 = reinterpret_cast<cx<double>*>(smem + L.off_sup);
-    int* ss = reinterpret_cast<int*>(smem + L.off_sup + MAX_SUPPORT * 16);
-    int* seg = ss + MAX_SUPPORT;
+    int* ss = reinterpret_cast<int*>(smem + L.off_sup + (size_t)L.nsw * 16);
+    int* seg = ss + L.nsw;
     uint16_t* ent = reinterpret_cast<uint16_t*>(seg + 16);
     const int nent = nd * nbe;
 
@@ -1319,12 +1354,12 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     // build reads the phases from the step table itself.
     constexpr bool DB = sep_nbuf(NBP) == 2;
     cx<double>* wv = reinterpret_cast<cx<double>*>(smem + L.off_w);
-    double* efs = reinterpret_cast<double*>(smem + L.off_w + 2 * MAX_SUPPORT * 16);
+    double* efs = reinterpret_cast<double*>(smem + L.off_w + 2 * (size_t)L.nsw * 16);
     auto weights = [&](int ti) {
         const double* st = ip.steptab + (size_t)ti * d.stride;
         const int sl = ti & 1;
         for (int q = tid; q < d.ns; q += bd)
-            wv[sl * MAX_SUPPORT + q] = cmul(sw1[q], ld<double>(st + STEP_HEAD + 2 * (ss[q] & 0xffff)));
+            wv[sl * L.nsw + q] = cmul(sw1[q], ld<double>(st + STEP_HEAD + 2 * (ss[q] & 0xffff)));
         for (int c = tid; c < 9; c += bd) efs[sl * 16 + c] = st[c];
     };
     for (int q = tid; q < nent; q += bd) {
@@ -1350,7 +1385,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
             for (int c = 0; c < 9; ++c) ef[c] = DB ? efs[(ti & 1) * 16 + c] : st[c];
         }
-        const cx<double>* wq = wv + (ti & 1) * MAX_SUPPORT;
+        const cx<double>* wq = wv + (ti & 1) * L.nsw;
         cx<T>* G = reinterpret_cast<cx<T>*>(smem + L.off_g + buf * L.gbytes);
         cx<double>* GV = reinterpret_cast<cx<double>*>(smem + L.off_gv + buf * L.gvbytes);
         for (int q = tid; q < nent; q += bd) {
@@ -1745,6 +1780,16 @@ __global__ __launch_bounds__(256) void k_elem_per_lane_batch(const DevInst* __re
                                                              const int2* __restrict__ tiles) {
     const int2 t = tiles[blockIdx.x];
     elem_body<EPL, T>(insts[t.x].d, insts[t.x].p, t.y);
+}
+// 33-64 bands: a 1024-lane workgroup per k-point, 4 elements per lane
+template <typename T>
+__global__ __launch_bounds__(1024) void k_elem_big(const SolveDesc d, const InstPtrs ip) {
+    elem_body<4, T, true>(d, ip, blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(1024) void k_elem_big_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    elem_body<4, T, true>(insts[t.x].d, insts[t.x].p, t.y);
 }
 
 // J[ti][omap[c]] = (sum over waves of component c, in a fixed order) / N;
@@ -2189,7 +2234,9 @@ size_t dev_bytes(const HostSolve& hs) {
     // propagation partials: at most 6 components per wave; elem tiles hold 256/G k in 4 waves
     const int E = d.nb * d.nb;
     const int G = lanes_per_k(d.nb);
-    const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64 : (size_t)((d.N + 256 / G - 1) / (256 / G)) * 4;
+    const int kpb = G >= 256 ? 1 : 256 / G;  // k-points per workgroup of max(256, G) lanes
+    const size_t waves = (d.nb <= 2) ? (size_t)(d.N + 63) / 64
+                                     : (size_t)((d.N + kpb - 1) / kpb) * (size_t)(std::max(256, G) / 64);
     // rho and its scratch twin (2 x 2 nb^2 N doubles); a device-side initial
     // state of 1-2 bands holds 3 more nb^2 N complex planes while it runs
     const size_t rho = 2 * (size_t)E * d.N;
@@ -2271,9 +2318,9 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = fal
     if (w <= 0) {
         const int chunks = (d.N2 + wmax * gpw - 1) / (wmax * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
-        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX) --w;
+        while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd, d.ns).bytes > LDS_MAX) --w;
     }
-    if (w < 1 || w > (lean ? SEP_LEAN_MAX : 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd).bytes > LDS_MAX)
+    if (w < 1 || w > (lean ? SEP_LEAN_MAX : 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd, d.ns).bytes > LDS_MAX)
         return k;
     k.kind = lean ? KIND_SEPL : KIND_SEP;
     k.a = nbp;
@@ -2293,7 +2340,7 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
     if ((k = key_bloch(ds, fp32)).kind) return k;
     if ((k = key_pair(ds, fp32)).kind) return k;
     if ((k = key_thread(ds, fp32)).kind) return k;
-    if (d.nb > 32) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 32)");
+    if (d.nb > 64) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 64)");
     if (batch && (k = key_sep(ds, fp32, 0, true)).kind) return k;
     if ((k = key_sep(ds, fp32, 0)).kind) {
         // a single solve whose lane groups carry one k-point each takes the
@@ -2302,10 +2349,14 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
         if (sep_kpw(d, k) == 1 && (kw = key_sep(ds, fp32, 0, true)).kind) return kw;
         return k;
     }
+    // large supports: the staged FP32 blocks exceed the LDS, the lean form has none
+    if ((k = key_sep(ds, fp32, 0, true)).kind) return k;
+    k = LaunchKey();
     k.kind = KIND_ELEM;
     k.fp32 = fp32;
     // elements per lane (lanes_per_k lanes per k-point: <= 64 up to 16 bands, 256 beyond)
     k.a = d.nb <= 8 ? 1 : d.nb <= 16 ? 4 : d.nb * d.nb <= 512 ? 2 : 4;
+    k.b = d.nb > 32 ? 1 : 0;  // the 1024-lane form
     return k;
 }
 
@@ -2387,8 +2438,8 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
     }
     if (k.kind == KIND_ELEM) {
         const int G = lanes_per_k(d.nb);
-        block = 256;
-        waves_per_tile = 4;
+        block = std::max(256, G);
+        waves_per_tile = block / 64;
         const int kpb = block / G;
         return (d.N + kpb - 1) / kpb;
     }
@@ -2399,12 +2450,13 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
 
 static size_t lds_of(const SolveDesc& d, const LaunchKey& k) {
     if (k.kind == KIND_SEP)
-        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd)
+        return sep_layout(k.a, k.c, k.fp32 ? 4 : 8, k.b, (k.fp32 && k.a <= 8) ? (size_t)d.ns * d.nb * d.nb : 0, d.nwd,
+                          d.ns)
             .bytes;
-    if (k.kind == KIND_SEPL) return sep_layout(k.a, k.c, 4, k.b, 0, d.nwd).bytes;
+    if (k.kind == KIND_SEPL) return sep_layout(k.a, k.c, 4, k.b, 0, d.nwd, d.ns).bytes;
     if (k.kind != KIND_ELEM) return 0;
     const int G = lanes_per_k(d.nb);
-    const int kpb = 256 / G;
+    const int kpb = std::max(256, G) / G;
     const size_t esz = k.fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
     return sizeof(cx<double>) * (size_t)kpb * d.ns + esz * (size_t)kpb * 2 * d.nb * d.nb;
 }
@@ -2463,6 +2515,10 @@ static void dispatch(const LaunchKey& k, F&& f) {
             if (k.a == 8 && k.c == 8) return f(k_sep_lean<8, 8>, k_sep_lean_batch<8, 8>);
             break;
         case KIND_ELEM:
+            if (k.b == 1) {
+                if (k.a != 4) break;
+                return go(k_elem_big<double>, k_elem_big_batch<double>, k_elem_big<float>, k_elem_big_batch<float>);
+            }
             if (k.a == 1)
                 return go(k_elem_per_lane<1, double>, k_elem_per_lane_batch<1, double>, k_elem_per_lane<1, float>,
                           k_elem_per_lane_batch<1, float>);

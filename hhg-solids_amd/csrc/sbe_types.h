@@ -19,7 +19,10 @@
 namespace hhg {
 
 enum { OP_H = 0, OP_X = 1, OP_V = 4, NOPS = 7 };
-constexpr int MAX_SUPPORT = 64;   // support points per instance (5x5 shell + window fits)
+// Support points per instance (a 15x15 shell and its window fit). The
+// per-support arrays of SolveDesc are narrow integers so that the descriptor
+// stays well inside the 4 KiB of kernel arguments.
+constexpr int MAX_SUPPORT = 256;
 constexpr int STEP_HEAD = 12;     // doubles before the phases in a steptab row
 
 // Role of a support point in the Hermitian-paired sums: when every block
@@ -33,19 +36,19 @@ enum { ROLE_ORIGIN = 0, ROLE_REP = 1, ROLE_PARTNER = 2 };
 // so the supports are grouped by their distinct n2 values.
 constexpr int SEP_MAXD = 8;
 
-struct SolveDesc {
+struct SolveDesc {  // a kernel argument: at most 4 KiB with PairDesc and InstPtrs
     int nb, N1, N2, N, nt, ns;
     int stride;                   // doubles per steptab row = STEP_HEAD + 2*ns
     int xmask, vmask;             // which rbc / v axes are live
     int herm;                     // 1 when the paired (Hermitian) sums are exact
-    int n1[MAX_SUPPORT], n2[MAX_SUPPORT];
-    uint32_t opmask[MAX_SUPPORT]; // bit o set when block (s, o) is non-zero
-    int role[MAX_SUPPORT];        // ROLE_* (meaningful when herm == 1)
+    int16_t n1[MAX_SUPPORT], n2[MAX_SUPPORT];
+    uint8_t opmask[MAX_SUPPORT];  // bit o set when block (s, o) is non-zero (NOPS = 7 bits)
+    uint8_t role[MAX_SUPPORT];    // ROLE_* (meaningful when herm == 1)
     int hermhx;                   // H and rbc blocks satisfy B(-n) = B(n)^dagger
     int nd;                       // distinct n2 values of the supports (0 when > SEP_MAXD)
     int dn2[SEP_MAXD];            // those values
     int dseg[SEP_MAXD + 1];       // supports of value d: sorder[dseg[d] .. dseg[d + 1])
-    int sorder[MAX_SUPPORT];
+    uint16_t sorder[MAX_SUPPORT];
     int nwd;                      // values d with a velocity block; wdi[d] = their rank or -1
     int wdi[SEP_MAXD];
 };
@@ -104,4 +107,7 @@ struct DevInst {
     double jcst[6];      // constant added to each component after the 1/N normalisation
 };
 
+
+static_assert(sizeof(SolveDesc) + sizeof(PairDesc) + sizeof(InstPtrs) <= 4096 - 64,
+              "descriptors must fit the kernel argument segment");
 }  // namespace hhg
