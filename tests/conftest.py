@@ -42,3 +42,30 @@ def oracle_lib():
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
     return oracle
+
+
+def hmcase0_closed_form_current(rho, A, N1, N2):
+    """J_a = (1/N) sum_k sum_ij V_a,ij(k - A) conj(rho_k,ij) with the physical
+    velocity of hmcase0 in closed form: H_01(R) = -2.42 eV at R = 0, a1, a2
+    (H_k[0][1] = -2.42 (1 + e^{-ik.a1} + e^{-ik.a2})), H_00 = -2.1 and H_11 =
+    2.1 on site, orbital centres x_A = 0 and x_B = 1.42028 Angstrom, so
+    V_01(q) = sum_R i (R + tau_A - tau_B) H_01(R) e^{-iq.R}
+            = -2.42 i [(tau_A - tau_B) f(q) + a1 e^{-iq.a1} + a2 e^{-iq.a2}],
+    f = 1 + e^{-iq.a1} + e^{-iq.a2}, V_10 = conj(V_01), V_00 = V_11 = 0, in eV*A.
+    This is the full velocity dH/dk plus the orbital-offset term; every R it
+    needs lies inside the 4x4 window of src/observable.cpp:16-30. rho: (N, 2, 2)
+    final states, A: (3,) vector potential (au) of the step."""
+    import numpy as np
+    a1 = np.array([2.130422493, -1.23, 0.0])
+    a2 = np.array([2.130422493, 1.23, 0.0])
+    tau = np.array([0.0 - 1.42028, 0.0, 0.0])  # tau_A - tau_B
+    bohr = 0.5291772
+    m1, m2 = np.divmod(np.arange(N1 * N2), N2)
+    e1 = np.exp(-2j * np.pi * m1 / N1) * np.exp(1j * (A @ a1) / bohr)
+    e2 = np.exp(-2j * np.pi * m2 / N2) * np.exp(1j * (A @ a2) / bohr)
+    f = 1 + e1 + e2
+    J = np.zeros(3, dtype=complex)
+    for a in range(3):
+        v01 = -2.42j * (tau[a] * f + a1[a] * e1 + a2[a] * e2)
+        J[a] = np.sum(v01 * np.conj(rho[:, 0, 1]) + np.conj(v01) * np.conj(rho[:, 1, 0]))
+    return J / (N1 * N2)

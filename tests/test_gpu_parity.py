@@ -160,6 +160,25 @@ def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model
         assert np.abs(Jref.imag).max() > 1e-6 * np.abs(Jref).max()  # the window current is complex here
 
 
+def test_device_current_is_the_closed_form_velocity_for_hmcase0(native):
+    """The README case (hmcase0, 100 x 100, 4096 steps, oblique polarisation):
+    the device's last current sample equals (1/N) sum_k Tr(v(k - A) rho_k) of
+    its own final states with hmcase0's velocity in closed form, pinning the
+    window, phases, units and the device reduction of J."""
+    from conftest import hmcase0_closed_form_current
+    prm = make_param(HMCASE0, nr=100, pol=(0.6, 0.8, 0.0), phi=(0.0, 0.5, 0.0))
+    with native.Solve(prm) as s:
+        s.run()
+        _, J = s.current()
+        rho = s.rho_k()
+        _, _, A = s.fields()
+        kind = s.kernel()
+    Jc = hmcase0_closed_form_current(rho, A[:, -1], 100, 100)
+    err = np.abs(Jc - J[-1]).max() / np.abs(J).max()
+    print(f"kernel {kind}: J[nt-1] vs closed form {err:.2e}")
+    assert err <= 1e-12
+
+
 def test_run_twice_continues_and_keeps_first_samples(native):
     """src/swe.cpp:183-223 + src/observable.cpp:29: a second run continues from
     the evolved state and appends; get_current still returns the first nt."""

@@ -226,6 +226,25 @@ def window_current(path, prm, rho, ti):
     return J / (N1 * N2)
 
 
+def test_window_current_is_the_closed_form_velocity_for_hmcase0(oracle_lib):
+    """hmcase0's hoppings all lie inside the 4x4 window, so the windowed current
+    of src/observable.cpp:16-30 must equal (1/N) sum_k Tr(v(k - A) rho_k) with
+    the physical velocity in closed form (conftest.hmcase0_closed_form_current);
+    a wrong window, phase sign, unit or orbital-offset term breaks this."""
+    from conftest import hmcase0_closed_form_current
+    prm = make_param(HMCASE0, nr=(8, 10), tmax=0.02197 * 70 + 1e-9, tmax_field=1.4, intensity=5e12,
+                     pol=(0.6, 0.8, 0.0), phi=(0.0, 0.3, 0.0))
+    nt = oracle_lib.nt(prm)
+    rho = oracle_lib.propagate_k(HMCASE0, prm, np.arange(80), nthreads=4)
+    _, _, A = oracle_lib.fields(prm)
+    _, J = oracle_lib.run(HMCASE0, prm, mode="perk", nthreads=4)
+    Jc = hmcase0_closed_form_current(rho, A[:, nt - 1], 8, 10)
+    assert np.abs(J[nt - 1, 0]) > 1e-6 and np.abs(J[nt - 1, 1]) > 1e-6
+    assert np.abs(Jc - J[nt - 1]).max() <= 1e-12 * np.abs(J[nt - 1]).max()
+    # the closed form sees the field: with A = 0 it differs
+    assert np.abs(hmcase0_closed_form_current(rho, np.zeros(3), 8, 10) - J[nt - 1]).max() > 1e-6 * np.abs(J).max()
+
+
 def test_propagate_k_closes_the_last_current(oracle_lib, tmp_path):
     """The final states of oracle_propagate_k over every k reproduce the last
     current sample of oracle_run through the window's k-space closed form,

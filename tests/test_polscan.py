@@ -49,3 +49,23 @@ def test_scan_equals_angles_alone(mode):
         _, _, hhg = polscan.spectra(J, p)
         assert np.array_equal(rec["data_sim"][i], hhg)  # every angle keeps its own spectra
     assert not np.allclose(rec["data_sim"][0], rec["data_sim"][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["batch", "serve"])
+def test_scan_spectra_vs_oracle(oracle_lib, mode):
+    """Every angle's spectra of a batched or served scan against the spectra of
+    the oracle's J(t) for the same angle (scripts/pol_scan.py:31-57 applied to
+    the FP64 oracle); the scan's currents are FP64, so the spectra agree to the
+    J parity bar."""
+    from conftest import HMCASE0
+    from hhgamd import polscan
+    prm = make_param(HMCASE0, nr=(12, 14), tmax=12.0, tmax_field=10.0, intensity=3e12)
+    rec = polscan.run(prm, n_angles=5, mode=mode)
+    for i, p in enumerate(polscan.params_for(prm, rec["pol_angles"])):
+        _, Jref = oracle_lib.run(HMCASE0, p, mode="perk")
+        _, _, ref = polscan.spectra(Jref, p)
+        scale = max(np.linalg.norm(ref[0]), np.linalg.norm(ref[1]))  # an axis may vanish by symmetry
+        for a in range(3):
+            err = np.linalg.norm(rec["data_sim"][i, a] - ref[a]) / scale
+            assert err <= 1e-10, (i, a, err)
