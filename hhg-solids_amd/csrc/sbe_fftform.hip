@@ -47,9 +47,13 @@ namespace fftform {
 // 128-byte row segments. Rows are contiguous: a row tile holds about 256
 // points (one line of 200-256, two of up to 128, four of up to 64), which
 // gives many small workgroups without tiny ones.
+// Lines of 257-512 points (the reference's default grid is 400 x 400) run in
+// tiles of one row or 4 columns.
 constexpr int TB_COLS = 8;
+constexpr int SHORT = 256;   // line capacity of the tiles of up to 8 lines
+constexpr int MAXLEN = 512;  // longest transform (LDS: (4 + 1) x 512 complex = 40 KiB)
 static int rows_per_tile(int L) { return L <= 64 ? 4 : L <= 128 ? 2 : 1; }
-constexpr int MAXLEN = 256;  // longest transform (LDS: (8 + 1) x MAXLEN complex = 36 KiB)
+static int cols_per_tile(int L) { return L <= SHORT ? TB_COLS : 4; }
 constexpr int MAXPASS = 12;
 
 struct Plan {
@@ -69,7 +73,7 @@ static Plan make_plan(int L) {
         }
     if (m != 1 || L > MAXLEN)
         throw std::runtime_error("FFT form: grid side " + std::to_string(L) +
-                                 " must factor into 2, 3, 5, 7, 11, 13 and be at most 256");
+                                 " must factor into 2, 3, 5, 7, 11, 13 and be at most 512");
     return p;
 }
 
@@ -86,10 +90,11 @@ __host__ __device__ constexpr int fft_threads(int TB) { return TB >= 8 ? 256 : T
 // e^{sign 2 pi i q k/(Ns R)} (k = j mod Ns), forms a size-R DFT and writes
 // outputs (j / Ns) Ns R + k + u Ns. Each thread holds the twiddled inputs of
 // its butterflies (at most NBF) in registers across a barrier, so one LDS
-// buffer suffices. w: the L roots e^{sign 2 pi i m / L}, in LDS.
-template <int R, int TB, int NT>
+// buffer suffices. w: the L roots e^{sign 2 pi i m / L}, in LDS. ML: the
+// longest line the tile holds (TB x ML points sets the butterflies per thread).
+template <int R, int TB, int NT, int ML>
 __device__ __forceinline__ void radix_pass(double2* buf, const double2* w, int L, int Ns) {
-    constexpr int NBF = (TB * MAXLEN / R + NT - 1) / NT;
+    constexpr int NBF = (TB * ML / R + NT - 1) / NT;
     const int M = L / R, total = TB * M, step = L / (Ns * R);
     double2 v[NBF][R];
 #pragma unroll
@@ -130,21 +135,22 @@ __device__ __forceinline__ void radix_pass(double2* buf, const double2* w, int L
     __syncthreads();
 }
 
-// All radix passes of the plan over the TB lines in buf (NT threads).
-template <int TB, int NT>
+// All radix passes of the plan over the TB lines of at most ML points in buf
+// (NT threads).
+template <int TB, int NT, int ML = SHORT>
 __device__ __forceinline__ void tile_passes(double2* buf, const double2* w, const Plan& plan) {
     const int L = plan.L;
     int Ns = 1;
     for (int p = 0; p < plan.n; ++p) {
         const int R = plan.r[p];
         switch (R) {
-            case 2: radix_pass<2, TB, NT>(buf, w, L, Ns); break;
-            case 3: radix_pass<3, TB, NT>(buf, w, L, Ns); break;
-            case 4: radix_pass<4, TB, NT>(buf, w, L, Ns); break;
-            case 5: radix_pass<5, TB, NT>(buf, w, L, Ns); break;
-            case 7: radix_pass<7, TB, NT>(buf, w, L, Ns); break;
-            case 11: radix_pass<11, TB, NT>(buf, w, L, Ns); break;
-            default: radix_pass<13, TB, NT>(buf, w, L, Ns); break;
+            case 2: radix_pass<2, TB, NT, ML>(buf, w, L, Ns); break;
+            case 3: radix_pass<3, TB, NT, ML>(buf, w, L, Ns); break;
+            case 4: radix_pass<4, TB, NT, ML>(buf, w, L, Ns); break;
+            case 5: radix_pass<5, TB, NT, ML>(buf, w, L, Ns); break;
+            case 7: radix_pass<7, TB, NT, ML>(buf, w, L, Ns); break;
+            case 11: radix_pass<11, TB, NT, ML>(buf, w, L, Ns); break;
+            default: radix_pass<13, TB, NT, ML>(buf, w, L, Ns); break;
         }
         Ns *= R;
     }
@@ -155,7 +161,8 @@ __device__ __forceinline__ void tile_passes(double2* buf, const double2* w, cons
 // workgroup. post: 0 store, 1 store -i x / N (the stage derivative),
 // 2 store x / N.
 // in and out may alias (in-place row pass): the tile is in LDS before any store.
-template <int TB>
+// ML: the longest line of the instance (SHORT, or MAXLEN for 1 row or 4 columns).
+template <int TB, int ML = SHORT>
 __global__ __launch_bounds__(fft_threads(TB), TB >= 4 ? 2 : 4) void k_fft_lines(const double2* in, double2* out,
                                                                int nch, int N1, int N2, int dim, int sign,
                                                                const double* __restrict__ tw, Plan plan, int post) {
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(fft_threads(TB), TB >= 4 ? 2 : 4) void k_fft_lines(
         if (line0 + l < lines) buf[l * L + i] = in[addr(l, i)];
     }
     __syncthreads();
-    tile_passes<TB, fft_threads(TB)>(buf, w, plan);
+    tile_passes<TB, fft_threads(TB), ML>(buf, w, plan);
     double2* src = buf;
     const double n = (double)N1 * (double)N2;
     for (int t = threadIdx.x; t < TB * L; t += blockDim.x) {
@@ -455,6 +462,17 @@ static void lines_pass(Solve* f, const double2* in, double2* out, int nch, int d
     const size_t lds = sizeof(double2) * (tb + 1) * L;
     const double* tw = dim == 1 ? f->tw2 : f->tw1;
     const Plan& p = dim == 1 ? f->p2 : f->p1;
+    if (L > SHORT) {  // long lines: one row or 4 columns per tile
+        if (tb == 1)
+            hipLaunchKernelGGL((k_fft_lines<1, MAXLEN>), dim3(blocks), dim3(threads), lds, f->stream, in, out, nch,
+                               f->N1, f->N2, dim, sign, tw, p, post);
+        else if (tb == 4)
+            hipLaunchKernelGGL((k_fft_lines<4, MAXLEN>), dim3(blocks), dim3(threads), lds, f->stream, in, out, nch,
+                               f->N1, f->N2, dim, sign, tw, p, post);
+        else
+            throw std::runtime_error("FFT form: lines over 256 points run one row or 4 columns per tile");
+        return;
+    }
     switch (tb) {
         case 1: hipLaunchKernelGGL(k_fft_lines<1>, dim3(blocks), dim3(threads), lds, f->stream, in, out, nch, f->N1,
                                    f->N2, dim, sign, tw, p, post); break;
@@ -470,14 +488,14 @@ static void lines_pass(Solve* f, const double2* in, double2* out, int nch, int d
 static void fft2(Solve* f, const double2* in, double2* out, int nch, int sign, int post) {
     // rows (length N2) in place, then columns (length N1) with the post-operation
     lines_pass(f, in, (double2*)in, nch, 1, sign, 0, rows_per_tile(f->N2));
-    lines_pass(f, in, out, nch, 0, sign, post, TB_COLS);
+    lines_pass(f, in, out, nch, 0, sign, post, cols_per_tile(f->N1));
 }
 
 // One RK4 stage in three launches (nb <= 4, a column of all 2 nb^2 fields in
 // LDS): forming + forward row FFTs, forward column FFTs + commutator + inverse
 // column FFTs, inverse row FFTs with the -i/N of the stage derivative.
-static bool fused_stage_ok(int nb, int N1) {
-    return nb <= 4 && sizeof(double2) * (size_t)(2 * nb * nb + 2) * N1 <= 160 * 1024;
+static bool fused_stage_ok(int nb, int N1, int N2) {
+    return nb <= 4 && N1 <= SHORT && N2 <= SHORT && sizeof(double2) * (size_t)(2 * nb * nb + 2) * N1 <= 160 * 1024;
 }
 
 static void fused_stage(Solve* f, int stride, int js, double coef, const double2* Kprev, double2* Kout) {
@@ -589,7 +607,7 @@ FFTFormSolve* fftform_create(const HostSolve& hs) {
                            d.N2, f->lat[0], f->lat[1], f->lat[2], f->lat[3], f->lat[4], f->lat[5], f->ph);
         const double coef[4] = {0.0, 0.5 * hs.dt, 0.5 * hs.dt, hs.dt};
         const int field[4] = {0, 1, 1, 2};
-        const bool fused = fused_stage_ok(d.nb, d.N1);
+        const bool fused = fused_stage_ok(d.nb, d.N1, d.N2);
         for (int s = 0; s < 4; ++s) {
             const double2* Kprev = s == 0 ? (const double2*)nullptr : f->K + (size_t)(s - 1) * plane;
             if (fused) {

@@ -111,7 +111,7 @@ void hhg_set_autotune(int on);
 int hhg_get_autotune(void);
 /* Unnormalised in-place 2-D FFTs on the GPU of nch row-major n1 x n2 complex
  * fields in host memory (sign -1 forward, +1 backward): the LDS-resident
- * mixed-radix transform of the FFT form (sides up to 256 with prime factors
+ * mixed-radix transform of the FFT form (sides up to 512 with prime factors
  * 2, 3, 5, 7, 11, 13). The transform of src/fftw_helper.cpp:5-33 without the
  * 1/N of ifft3. Returns 0, or -1 with hhg_last_error set. */
 int hhg_fft2(double* data, int nch, int n1, int n2, int sign);

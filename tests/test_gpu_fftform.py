@@ -12,7 +12,8 @@ FP64_TOL = 1e-10
 
 
 @pytest.mark.parametrize("shape", [(1, 8, 10), (3, 12, 30), (2, 200, 240), (2, 14, 26), (1, 256, 256), (4, 60, 4),
-                                   (2, 22, 66), (1, 143, 39), (3, 121, 169), (2, 49, 250), (1, 2, 3)])
+                                   (2, 22, 66), (1, 143, 39), (3, 121, 169), (2, 49, 250), (1, 2, 3),
+                                   (1, 400, 400), (2, 512, 270), (1, 288, 507), (2, 14, 484)])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft2_matches_numpy(shape, sign):
     from hhgamd import lib
@@ -25,7 +26,7 @@ def test_fft2_matches_numpy(shape, sign):
     assert err <= 1e-13
 
 
-@pytest.mark.parametrize("shape", [(8, 200, 200), (3, 12, 30), (2, 256, 256)])
+@pytest.mark.parametrize("shape", [(8, 200, 200), (3, 12, 30), (2, 256, 256), (2, 400, 400)])
 def test_fft2_plan_on_device_memory(shape):
     """hhg_fft2_plan_exec on device memory: forward against numpy, then the
     unnormalised inverse returns N times the input; repeats chain."""
@@ -106,6 +107,18 @@ def test_fftform_matches_per_k_kernels(tmp_path):
         err = rel_l2(Jf, Jk)
         print(f"{nr}: FFT form vs per-k {err:.2e}")
         assert err <= FP64_TOL
+
+
+def test_fftform_at_the_reference_default_grid():
+    """400 x 400 (include/settings.h:4): lines over 256 points run one row or
+    four columns per tile and the stages take the unfused path; against the
+    per-k kernel on a short pulse."""
+    p = make_param(HALDANE, nr=(400, 400), tmax=1.2, tmax_field=1.0, pol=(0.6, 0.8, 0.0), intensity=4e12)
+    k, Jf = _run(p, "fftform")
+    _, Jk = _run(p, "auto")
+    err = rel_l2(Jf, Jk)
+    print(f"400 x 400: FFT form vs per-k {err:.2e}")
+    assert k == 7 and err <= FP64_TOL
 
 
 def test_fftform_continues_and_rejects_fp32():
