@@ -24,6 +24,8 @@
 //    step index from a device counter that the last node advances.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
@@ -437,6 +439,7 @@ struct Solve {
     double2 *ph = nullptr, *Hd = nullptr, *Xd = nullptr, *V = nullptr, *rho = nullptr, *K = nullptr,
             *F = nullptr, *C = nullptr, *win = nullptr;
     float last_ms = 0.f;
+    bool eager = false;  // enqueue each step's kernels directly instead of replaying the graph
 };
 
 template <typename T>
@@ -542,6 +545,36 @@ static void destroy(Solve* f) {
     delete f;
 }
 
+// The kernels of one RK4 step on f's stream (captured once as a graph, or
+// enqueued per step in eager mode): phase, 4 stages, update with the
+// observable wrap, window current (which advances the device step counter).
+static void enqueue_step(Solve* f) {
+    const int nb2 = f->nb * f->nb, N = f->N;
+    const size_t plane = (size_t)nb2 * N;
+    const int blk = 256;
+    const unsigned gp = (unsigned)((plane + blk - 1) / blk);
+    hipLaunchKernelGGL(k_phase, dim3((N + blk - 1) / blk), dim3(blk), 0, f->stream, f->step, f->A, f->nt, f->N1,
+                       f->N2, f->lat[0], f->lat[1], f->lat[2], f->lat[3], f->lat[4], f->lat[5], f->ph);
+    const double coef[4] = {0.0, 0.5 * f->dt, 0.5 * f->dt, f->dt};
+    const int field[4] = {0, 1, 1, 2};
+    const bool fused = fused_stage_ok(f->nb, f->N1, f->N2);
+    for (int s = 0; s < 4; ++s) {
+        const double2* Kprev = s == 0 ? (const double2*)nullptr : f->K + (size_t)(s - 1) * plane;
+        if (fused) {
+            fused_stage(f, f->stride, field[s], coef[s], Kprev, f->K + (size_t)s * plane);
+            continue;
+        }
+        hipLaunchKernelGGL(k_form, dim3(gp), dim3(blk), 0, f->stream, f->step, f->steptab, f->stride, field[s],
+                           coef[s], nb2, N, f->ph, f->Hd, f->Xd, f->rho, Kprev, f->F);
+        fft2(f, f->F, f->F, 2 * nb2, -1, 0);
+        hipLaunchKernelGGL(k_comm, dim3((N + blk - 1) / blk), dim3(blk), 0, f->stream, f->nb, N, f->F, f->C);
+        fft2(f, f->C, f->K + (size_t)s * plane, nb2, +1, 1);
+    }
+    hipLaunchKernelGGL(k_update, dim3(gp), dim3(blk), 0, f->stream, nb2, N, f->N1, f->N2, 0.1666666666666 * f->dt,
+                       f->ph, f->K, f->rho, f->win);
+    hipLaunchKernelGGL(k_current, dim3(1), dim3(64), 0, f->stream, f->step, nb2, f->V, f->win, f->J);
+}
+
 }  // namespace fftform
 
 using namespace fftform;
@@ -599,32 +632,15 @@ FFTFormSolve* fftform_create(const HostSolve& hs) {
         // initial state rho(R) = ifftshift(IFFT(rho_k)) / N in FFT order
         fft2(f, f->C, f->rho, nb2, +1, 2);
         FF_CHECK(hipGetLastError());
-        // one RK4 step as a graph
-        FF_CHECK(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
-        const int blk = 256;
-        const unsigned gp = (unsigned)((plane + blk - 1) / blk);
-        hipLaunchKernelGGL(k_phase, dim3((N + blk - 1) / blk), dim3(blk), 0, f->stream, f->step, f->A, d.nt, d.N1,
-                           d.N2, f->lat[0], f->lat[1], f->lat[2], f->lat[3], f->lat[4], f->lat[5], f->ph);
-        const double coef[4] = {0.0, 0.5 * hs.dt, 0.5 * hs.dt, hs.dt};
-        const int field[4] = {0, 1, 1, 2};
-        const bool fused = fused_stage_ok(d.nb, d.N1, d.N2);
-        for (int s = 0; s < 4; ++s) {
-            const double2* Kprev = s == 0 ? (const double2*)nullptr : f->K + (size_t)(s - 1) * plane;
-            if (fused) {
-                fused_stage(f, d.stride, field[s], coef[s], Kprev, f->K + (size_t)s * plane);
-                continue;
-            }
-            hipLaunchKernelGGL(k_form, dim3(gp), dim3(blk), 0, f->stream, f->step, f->steptab, d.stride, field[s],
-                               coef[s], nb2, N, f->ph, f->Hd, f->Xd, f->rho, Kprev, f->F);
-            fft2(f, f->F, f->F, 2 * nb2, -1, 0);
-            hipLaunchKernelGGL(k_comm, dim3((N + blk - 1) / blk), dim3(blk), 0, f->stream, d.nb, N, f->F, f->C);
-            fft2(f, f->C, f->K + (size_t)s * plane, nb2, +1, 1);
+        // one RK4 step as a graph (HHG_FFTFORM_EAGER=1: enqueued directly every step)
+        const char* ev = std::getenv("HHG_FFTFORM_EAGER");
+        f->eager = ev && ev[0] == '1';
+        if (!f->eager) {
+            FF_CHECK(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(f);
+            FF_CHECK(hipStreamEndCapture(f->stream, &f->graph));
+            FF_CHECK(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
         }
-        hipLaunchKernelGGL(k_update, dim3(gp), dim3(blk), 0, f->stream, nb2, N, d.N1, d.N2,
-                           0.1666666666666 * hs.dt, f->ph, f->K, f->rho, f->win);
-        hipLaunchKernelGGL(k_current, dim3(1), dim3(64), 0, f->stream, f->step, nb2, f->V, f->win, f->J);
-        FF_CHECK(hipStreamEndCapture(f->stream, &f->graph));
-        FF_CHECK(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
         FF_CHECK(hipStreamSynchronize(f->stream));
     } catch (...) {
         destroy(f);
@@ -637,7 +653,14 @@ double fftform_run(FFTFormSolve* fs, double* J_out) {
     Solve* f = reinterpret_cast<Solve*>(fs);
     hipLaunchKernelGGL(k_set, dim3(1), dim3(1), 0, f->stream, f->step, 0);
     FF_CHECK(hipEventRecord(f->e0, f->stream));
-    for (int t = 0; t < f->nt; ++t) FF_CHECK(hipGraphLaunch(f->exec, f->stream));
+    for (int t = 0; t < f->nt; ++t) {
+        if (f->eager) {
+            enqueue_step(f);
+            FF_CHECK(hipGetLastError());
+        } else {
+            FF_CHECK(hipGraphLaunch(f->exec, f->stream));
+        }
+    }
     FF_CHECK(hipEventRecord(f->e1, f->stream));
     FF_CHECK(hipMemcpyAsync(J_out, f->J, sizeof(double) * 6 * (size_t)f->nt, hipMemcpyDeviceToHost, f->stream));
     FF_CHECK(hipStreamSynchronize(f->stream));

@@ -79,7 +79,10 @@ void SWESim_set_precision(SWESim* swe, int precision);
  * 1 = generic kernels, 2 = the reference's real-space algorithm (rho(R), 2-D
  * FFTs of every stage in LDS-resident radix kernels, one RK4 step replayed
  * as a HIP graph; FP64 only; it keeps its own state, started from the
- * initial state on its first run). */
+ * initial state on its first run). With HHG_FFTFORM_EAGER=1 in the
+ * environment at that first run the FFT form enqueues each step's kernels
+ * directly instead of replaying the graph (same kernels and order; for
+ * tools that trace kernels launched one by one). */
 void SWESim_set_kernel_variant(SWESim* swe, int variant);
 /* Kernel the next run launches: 1 = thread per k, 2 = Hermitian nb=2 with
  * register pairs, 3 = element per lane, 4 = Hermitian nb=2 with streamed

@@ -121,6 +121,15 @@ def test_fftform_at_the_reference_default_grid():
     assert k == 7 and err <= FP64_TOL
 
 
+def test_fftform_eager_steps_equal_graph_replay(monkeypatch):
+    """HHG_FFTFORM_EAGER=1: the step's kernels enqueued directly give bitwise the graph replay's J."""
+    p = make_param(HALDANE, nr=(16, 20), tmax=6.0, tmax_field=5.0, pol=(0.6, 0.8, 0.0), intensity=3e12)
+    _, Jg = _run(p, "fftform")
+    monkeypatch.setenv("HHG_FFTFORM_EAGER", "1")
+    _, Je = _run(p, "fftform")
+    assert np.array_equal(Je, Jg)
+
+
 def test_fftform_continues_and_rejects_fp32():
     from hhgamd import lib
     p = make_param(HMCASE0, nr=(8, 8), tmax=4.0, tmax_field=3.0)
