@@ -235,6 +235,16 @@ class Solve:
         self._check("run_simulation")
         return self.lib.SWESim_last_run_ms(self.ptr)
 
+    def restart(self):
+        """Re-initialise: equilibrium state, current history cleared (SWESim_restart)."""
+        self.lib.SWESim_restart(self.ptr)
+        self._check("restart")
+
+    def set_precision(self, precision):
+        """Stage arithmetic of the next runs: "fp64" or "fp32"."""
+        self.lib.SWESim_set_precision(self.ptr, {"fp64": 0, "fp32": 1}.get(precision, -1))
+        self._check("set_precision")
+
     def kernel(self):
         return self.lib.SWESim_get_kernel(self.ptr)
 

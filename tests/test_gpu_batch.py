@@ -104,3 +104,37 @@ def test_batch_with_element_kernel_members(tmp_path, oracle_lib):
         assert rel_l2(Jb, Ja) <= SAME_TOL
         assert rel_l2(Jb, Jref) <= FP64_TOL
         s.close()
+
+
+def test_member_changes_after_adding(tmp_path):
+    """A batch resolves its members at every run: a member restarted after it
+    was added starts again from equilibrium, a member switched to FP32 stages
+    runs them, and a deleted member leaves the batch."""
+    from hhgamd import lib
+    params = _mixed_params(tmp_path)
+    pa, pb, pc = params[0], params[5], params[1]   # Haldane 16x20, 4-band 8x8, hmcase0 12x12
+    with lib.Solve(pa) as s:
+        s.run()
+        Ja = s.current()[1]
+    with lib.Solve(pb) as s:
+        s.run()
+        s.set_precision("fp32")
+        s.run()
+        Jb = s.current_run(-1)[1]
+    A, B, C = lib.Solve(pa), lib.Solve(pb), lib.Solve(pc)
+    b = lib.Batch([A, B, C])
+    try:
+        b.run()
+        A.restart()
+        B.set_precision("fp32")
+        C.close()
+        assert b.lib.hhg_batch_size(b.ptr) == 2
+        b.run()
+        ea, eb = rel_l2(A.current()[1], Ja), rel_l2(B.current_run(-1)[1], Jb)
+        print(f"restarted member vs a fresh solo run {ea:.1e}; FP32-switched member vs the same switch solo {eb:.1e}")
+        assert A.current()[1].shape == Ja.shape and ea <= SAME_TOL
+        assert eb <= SAME_TOL
+    finally:
+        b.close()
+        A.close()
+        B.close()

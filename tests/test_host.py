@@ -167,15 +167,15 @@ def test_server_failure_paths_without_gpu(native, tmp_path):
 def test_fft2_rejects_unsupported_sizes_without_gpu(native):
     with pytest.raises(RuntimeError, match="factor"):
         native.fft2(np.zeros((17, 8), dtype=complex))
-    with pytest.raises(RuntimeError, match="at most 256"):
-        native.fft2(np.zeros((8, 512), dtype=complex))
+    with pytest.raises(RuntimeError, match="at most 512"):
+        native.fft2(np.zeros((8, 1024), dtype=complex))
 
 
 def test_fft2_plan_rejects_unsupported_sizes_without_gpu(native):
     with pytest.raises(RuntimeError, match="factor"):
         native.FFT2Plan(2, 17, 8)
-    with pytest.raises(RuntimeError, match="at most 256"):
-        native.FFT2Plan(1, 8, 512)
+    with pytest.raises(RuntimeError, match="at most 512"):
+        native.FFT2Plan(1, 8, 1024)
     with pytest.raises(RuntimeError, match="nch"):
         native.FFT2Plan(0, 8, 8)
 
