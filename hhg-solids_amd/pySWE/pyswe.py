@@ -16,10 +16,8 @@ import os
 
 import numpy as np
 
-ND_POINTER_1_int32 = np.ctypeslib.ndpointer(dtype=np.int32, ndim=1, flags="C")
-ND_POINTER_1_double = np.ctypeslib.ndpointer(dtype=np.float64, ndim=1, flags="C")
-ND_POINTER_1_float = np.ctypeslib.ndpointer(dtype=np.float32, ndim=1, flags="C")
-ND_POINTER_1_complex = np.ctypeslib.ndpointer(dtype=np.complex128, ndim=1, flags="C")
+_F64 = np.ctypeslib.ndpointer(dtype=np.float64, ndim=1, flags="C")
+_C128 = np.ctypeslib.ndpointer(dtype=np.complex128, ndim=1, flags="C")
 
 DEFAULT_LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libwannier.so")
 _PRECISION = {"fp64": 0, "fp32": 1}
@@ -62,8 +60,7 @@ class SWE:
         self.swe_cpp.SWESim_init.restype = None
         self.swe_cpp.SWESim_restart.argtypes = [ct.c_void_p]
         self.swe_cpp.SWESim_restart.restype = None
-        self.swe_cpp.SWESim_get_current.argtypes = [ct.c_void_p, ND_POINTER_1_double, ND_POINTER_1_complex,
-                                                    ND_POINTER_1_complex, ND_POINTER_1_complex]
+        self.swe_cpp.SWESim_get_current.argtypes = [ct.c_void_p, _F64, _C128, _C128, _C128]
         self.swe_cpp.SWESim_get_current.restype = None
         self.swe_cpp.SWESim_delete.argtypes = [ct.c_void_p]
         self.swe_cpp.SWESim_delete.restype = None
@@ -140,7 +137,7 @@ class Settings:
             getattr(self.settings_swe, "Settings_set_" + name).argtypes = [ct.c_void_p, ct.c_double]
             getattr(self.settings_swe, "Settings_set_" + name).restype = None
         for name in ("pol_vec", "phi_vec"):
-            getattr(self.settings_swe, "Settings_set_" + name).argtypes = [ct.c_void_p, ND_POINTER_1_double]
+            getattr(self.settings_swe, "Settings_set_" + name).argtypes = [ct.c_void_p, _F64]
             getattr(self.settings_swe, "Settings_set_" + name).restype = None
 
         self.ptr = self.settings_swe.Settings_new()
