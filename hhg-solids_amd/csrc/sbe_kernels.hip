@@ -1646,27 +1646,57 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         }
         }
         // current of this step, one pass over GV for all k-points p of the lane
-        // group: sum_p sum_d tw2[m2_p n2_d] sum_ij GV[d][a][ij] conj(rho_p,ij)
+        // group: sum_d sum_ij GV[d][a][ij] P_d,ij with the twiddles applied
+        // first, P_d,ij = sum_p tw2[m2_p n2_d] conj(rho_p,ij), so each axis costs
+        // one complex MAC per element, not one per element and k-point (8 bands,
+        // two k-points per group: 120 x 120 FP32 50.4 -> 47.7 ms, FP64 72.9 ->
+        // 69.9 ms). One code path per kernel: keeping the per-k order beside it
+        // for groups that carry one k-point at run time made the FP32 stage
+        // block slower (53.3 ms); kernels of one k-point per group keep it.
 #pragma unroll
         for (int dd = 0; dd < MAXD; ++dd) {
             if (!((wmask >> dd) & 1)) continue;  // no velocity block at this n2 (uniform)
+            if constexpr (KPW_MAX == 1) {
+                // one k-point per group (4 bands): the twiddle after the sum
+                const cx<double> zz = zval(0, dd);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (!((d.vmask >> a) & 1)) continue;
-                const cx<double>* vp = GV + (size_t)(wrank(dd) * 3 + a) * E;
-                cx<double> v0, v1, v2, v3;
-                TD::ld2(vp, i0, j0, v0, v1);
-                TD::ld2(vp, i0 + 1, j0, v2, v3);
+                for (int a = 0; a < 3; ++a) {
+                    if (!((d.vmask >> a) & 1)) continue;
+                    const cx<double>* vp = GV + (size_t)(wrank(dd) * 3 + a) * E;
+                    cx<double> v0, v1, v2, v3;
+                    TD::ld2(vp, i0, j0, v0, v1);
+                    TD::ld2(vp, i0 + 1, j0, v2, v3);
+                    cx<double> s = {0, 0};
+                    cfmac(s, rhoA[0][0], v0);
+                    cfmac(s, rhoA[0][1], v1);
+                    cfmac(s, rhoA[0][2], v2);
+                    cfmac(s, rhoA[0][3], v3);
+                    cfma(jv[a], zz, s);
+                }
+            } else {
+                cx<double> P[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
                 for (int p = 0; p < KPW_MAX; ++p) {
                     if (p < KPW) {
-                        cx<double> s = {0, 0};
-                        cfmac(s, rhoA[p][0], v0);
-                        cfmac(s, rhoA[p][1], v1);
-                        cfmac(s, rhoA[p][2], v2);
-                        cfmac(s, rhoA[p][3], v3);
-                        cfma(jv[a], zval(p, dd), s);
+                        const cx<double> zz = zval(p, dd);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) cfmac(P[q], rhoA[p][q], zz);
                     }
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (!((d.vmask >> a) & 1)) continue;
+                    const cx<double>* vp = GV + (size_t)(wrank(dd) * 3 + a) * E;
+                    cx<double> v0, v1, v2, v3;
+                    TD::ld2(vp, i0, j0, v0, v1);
+                    TD::ld2(vp, i0 + 1, j0, v2, v3);
+                    cx<double> s = {0, 0};  // a short chain per (d, a)
+                    cfma(s, v0, P[0]);
+                    cfma(s, v1, P[1]);
+                    cfma(s, v2, P[2]);
+                    cfma(s, v3, P[3]);
+                    jv[a].re += s.re;
+                    jv[a].im += s.im;
                 }
             }
         }
@@ -2950,15 +2980,16 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     if (kind == KIND_SEP || kind == KIND_SEPL) {
         // padded size P, table depth D (MAXD): stage sums 3 D P^2 cMAC,
         // products 4 P^3 cMAC, the M - M^dagger combination and RK4 axpys
-        // ~26 P^2 flops, current 3 D (4 + 1) cMAC per 2x2 lane block; plus the
-        // per-row tables (90 flops per support and element) amortised over
-        // the k-points of a workgroup
+        // ~26 P^2 flops, current per 2x2 lane block and value d 4 cMAC for the
+        // twiddled state and 3 x 4 cMAC shared by the KPW k-points of the
+        // group (3 (4 + 1) cMAC per k-point for 4 bands); plus the per-row tables (90 flops per support and element)
+        // amortised over the k-points of a workgroup
         const LaunchKey k = classify(ds, precision);
-        const double P = k.a, P2 = P * P, D = k.c;
-        const double kpt = k.b * (64.0 / ((P / 2) * (P / 2))) * sep_kpw(d, k);
+        const double P = k.a, P2 = P * P, D = k.c, kpw = sep_kpw(d, k);
+        const double kpt = k.b * (64.0 / ((P / 2) * (P / 2))) * kpw;
         const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);
         const double table = 90.0 * d.ns * d.nb * d.nb;
-        return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + 3 * D * (P2 / 4) * 40.0 + table / (kpt * used);
+        return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + D * (P2 / 4) * (P == 4 ? 120.0 : 32.0 + 96.0 / kpw) + table / (kpt * used);
     }
     const double nb = d.nb;
     // element kernel: one nb^3 product per stage for Hermitian models, two otherwise
