@@ -1180,6 +1180,10 @@ __host__ __device__ constexpr int sep_nbuf(int NBP) { return NBP <= 8 ? 2 : 1; }
 // Most k-points a wave carries through a step under one table build (the
 // launch picks 1 .. this, a power of two)
 __host__ __device__ constexpr int sep_kpw_max(int NBP) { return NBP == 16 ? 4 : NBP == 8 ? 2 : 1; }
+// 3-4 bands in FP32: the commutator on the matrix cores (v_mfma_f32_4x4x1f32,
+// 16 blocks of 4 lanes). Each lane owns one column of its k-point (4 lanes =
+// one block) and HX - XH is formed from registers, with no LDS exchange.
+__host__ __device__ constexpr bool sep_mf(int NBP, int tsize) { return NBP == 4 && tsize == 4; }
 // nstage: support blocks x elements whose H and rbc blocks are staged in LDS
 // (FP32 path, NBP <= 8: ns nb^2), 0 when the table build reads them from HBM.
 // nwd: values d with a velocity block; for NBP = 16, where the LDS bounds the
@@ -1190,6 +1194,7 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     const size_t E = (size_t)NBP * NBP;
     const int GPW = 64 / ((NBP / 2) * (NBP / 2));
     const int nbuf = sep_nbuf(NBP);
+    const bool mf = sep_mf(NBP, tsize);
     size_t o = 0;
     L.gbytes = 3 * (size_t)MAXD * E * 2 * tsize;
     L.off_g = o; o = align16(o + nbuf * L.gbytes);
@@ -1199,7 +1204,9 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     if (nbuf == 2) { L.off_w = o; o = align16(o + 2 * ((size_t)L.nsw * 16 + 16 * 8)); }  // weights + field rows
     else L.off_w = 0;
     L.off_b = o; o = align16(o + nstage * 4 * 8);
-    L.kbytes = 2 * E * 2 * tsize + 64;  // H/M and X of one k-point; the pad staggers banks
+    // H/M and X of one k-point; the pad staggers banks (none in the matrix-core
+    // form, whose commutator stays in registers)
+    L.kbytes = mf ? 0 : 2 * E * 2 * tsize + 64;
     L.off_k = o; o = align16(o + (size_t)W * GPW * L.kbytes);
     // row twiddles of the k-points of each wave when a wave may carry several
     const int kmax = sep_kpw_max(NBP);
@@ -1255,6 +1262,34 @@ struct Tile<double, NBP> {
     }
 };
 
+// Tiles of the 4-band matrix-core form, where each lane owns one column of
+// its k-point. FP32 (stage tables): 16-byte chunks of two rows (i, i + 1) of a
+// column, laid out [row pair][column]; FP64 (velocity tables): row-major.
+template <typename T, int NBP>
+struct SegTile {
+    __host__ __device__ static int off(int i, int j) {
+        if constexpr (std::is_same<T, float>::value) return 2 * (NBP * (i >> 1) + j) + (i & 1);
+        else return i * NBP + j;
+    }
+    // rows i0 .. i0 + 3 of column j (i0 even)
+    __device__ static void ld4(const cx<T>* t, int i0, int j, cx<T> (&v)[4]) {
+        if constexpr (std::is_same<T, float>::value) {
+#pragma unroll
+            for (int q = 0; q < 4; q += 2) {
+                const float4 a = *reinterpret_cast<const float4*>(t + off(i0 + q, j));
+                v[q] = {a.x, a.y};
+                v[q + 1] = {a.z, a.w};
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double2 a = *reinterpret_cast<const double2*>(t + off(i0 + q, j));
+                v[q] = {a.x, a.y};
+            }
+        }
+    }
+};
+
 template <typename T>
 __device__ __forceinline__ void cfma(cx<T>& acc, cx<T> a, cx<T> b) {  // acc += a*b
     acc.re = fmaT(a.re, b.re, fmaT(-a.im, b.im, acc.re));
@@ -1298,8 +1333,10 @@ __device__ __forceinline__ double wave_total(double v) {
 template <int NBP, int MAXD, typename T, bool LEAN = false>
 __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip, int tile) {
     constexpr int HB = NBP / 2, LPK = HB * HB, GPW = 64 / LPK, E = NBP * NBP;
-    using TT = Tile<T, NBP>;
-    using TD = Tile<double, NBP>;
+    // MF: the 4-band FP32 matrix-core form, a column per lane
+    constexpr bool MF = sep_mf(NBP, (int)sizeof(T));
+    using TT = typename std::conditional<MF, SegTile<T, NBP>, Tile<T, NBP>>::type;
+    using TD = typename std::conditional<MF, SegTile<double, NBP>, Tile<double, NBP>>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     const int W = blockDim.x >> 6;
     // FP32 path, NBP <= 8: the H and rbc blocks are staged once in LDS as FP32
@@ -1310,6 +1347,9 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
     const int wave = tid >> 6, lane = tid & 63;
     const int g = lane / LPK, bl = lane - g * LPK;
     const int i0 = 2 * (bl / HB), j0 = 2 * (bl - (bl / HB) * HB);
+    // element q = 0..3 of the lane: 2x2 block (i0, j0), or (MF) row q of column bl
+    auto eli = [&](int q) { return MF ? q : i0 + (q >> 1); };
+    auto elj = [&](int q) { return MF ? bl : j0 + (q & 1); };
     // KPW k-points per lane group (ip.kpw, 1 .. sep_kpw_max(NBP)) share each
     // table build; k-point p of group g of wave w sits at m2 = base + (p W + w) GPW + g
     constexpr int KPW_MAX = sep_kpw_max(NBP);
@@ -1456,7 +1496,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         const int k = m1 * N2 + (kval ? m2 : 0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+            const int ei = eli(q), ej = elj(q);
             rhoA[p][q] = (kval && ei < nb && ej < nb) ? ld<double>(ip.rho_in + 2 * ((size_t)(ei * nb + ej) * N + k))
                                                       : cx<double>{0.0, 0.0};
         }
@@ -1528,8 +1568,14 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 const cx<T> zz = zk(dd);
                 const cx<T>* gp = G + (size_t)(j * MAXD + dd) * E;
                 cx<T> g0, g1, g2, g3;
-                TT::ld2(gp, i0, j0, g0, g1);
-                TT::ld2(gp, i0 + 1, j0, g2, g3);
+                if constexpr (MF) {
+                    cx<T> gc[4];
+                    TT::ld4(gp, 0, bl, gc);
+                    g0 = gc[0]; g1 = gc[1]; g2 = gc[2]; g3 = gc[3];
+                } else {
+                    TT::ld2(gp, i0, j0, g0, g1);
+                    TT::ld2(gp, i0 + 1, j0, g2, g3);
+                }
                 cfma(Hs[0], zz, g0);
                 cfma(Hs[1], zz, g1);
                 cfma(Hs[2], zz, g2);
@@ -1537,6 +1583,32 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             }
         };
         auto deriv = [&]() {
+            if constexpr (MF) {
+                // K = -i C with C = H X - X H, on v_mfma_f32_4x4x1f32: the 4 lanes of
+                // a k-point form one block, lane 4b + i supplies A[i], lane 4b + j B[j],
+                // and lane 4b + j receives D[r][j], r = 0..3 (its own column). With
+                // H, X Hermitian, C_ij = sum_m conj(H_mi) X_mj - conj(X_mi) H_mj, so
+                // both operands are the lanes' own column elements (m = 0..3):
+                //   Re C = sum_m (Hr_i Xr_j + Hi_i Xi_j) - (Xr_i Hr_j + Xi_i Hi_j)
+                //   Im C = sum_m (Hr_i Xi_j + Xi_i Hr_j) - (Hi_i Xr_j + Xr_i Hi_j)
+                // four accumulators of 8 rank-1 updates each (32 MFMAs per stage).
+                f32x4 rp = {0, 0, 0, 0}, rn = {0, 0, 0, 0}, ip1 = {0, 0, 0, 0}, in1 = {0, 0, 0, 0};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float hr = Hs[m].re, hi = Hs[m].im, xr = rs[m].re, xi = rs[m].im;
+                    rp = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xr, rp, 0, 0, 0);
+                    rn = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hr, rn, 0, 0, 0);
+                    ip1 = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xi, ip1, 0, 0, 0);
+                    in1 = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xr, in1, 0, 0, 0);
+                    rp = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xi, rp, 0, 0, 0);
+                    rn = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hi, rn, 0, 0, 0);
+                    ip1 = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hr, ip1, 0, 0, 0);
+                    in1 = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hi, in1, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) K[q] = {ip1[q] - in1[q], rn[q] - rp[q]};
+                return;
+            } else {
             TT::st2(bufH, i0, j0, Hs[0], Hs[1]);
             TT::st2(bufH, i0 + 1, j0, Hs[2], Hs[3]);
             TT::st2(bufX, i0, j0, rs[0], rs[1]);
@@ -1603,6 +1675,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             K[2] = {M[2].im + t1.im, t1.re - M[2].re};
             K[3] = {M[3].im + u1.im, u1.re - M[3].re};
             wave_lds_sync();
+            }
         };
         hstage(0);
 #pragma unroll
@@ -1664,8 +1737,14 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                     if (!((d.vmask >> a) & 1)) continue;
                     const cx<double>* vp = GV + (size_t)(wrank(dd) * 3 + a) * E;
                     cx<double> v0, v1, v2, v3;
-                    TD::ld2(vp, i0, j0, v0, v1);
-                    TD::ld2(vp, i0 + 1, j0, v2, v3);
+                    if constexpr (MF) {
+                        cx<double> vc[4];
+                        TD::ld4(vp, 0, bl, vc);
+                        v0 = vc[0]; v1 = vc[1]; v2 = vc[2]; v3 = vc[3];
+                    } else {
+                        TD::ld2(vp, i0, j0, v0, v1);
+                        TD::ld2(vp, i0 + 1, j0, v2, v3);
+                    }
                     cx<double> s = {0, 0};
                     cfmac(s, rhoA[0][0], v0);
                     cfmac(s, rhoA[0][1], v1);
@@ -1728,7 +1807,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
         const int k = m1 * N2 + (kval ? m2 : 0);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int ei = i0 + (q >> 1), ej = j0 + (q & 1);
+            const int ei = eli(q), ej = elj(q);
             if (kval && ei < nb && ej < nb) {
                 ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k)] = rhoA[p][q].re;
                 ip.rho_out[2 * ((size_t)(ei * nb + ej) * N + k) + 1] = rhoA[p][q].im;
@@ -2329,13 +2408,17 @@ static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
     k.b = d.ns <= 8 ? 8 : d.ns <= 16 ? 16 : 32;  // MAXU: every support point must fit
     return k;
 }
+static int device_cus();
 // w <= 0: waves per workgroup by the default rule (equal chunks of a k-row,
-// at most 8 waves, reduced until the LDS footprint fits)
+// at most 8 waves, reduced until the LDS footprint fits). fill (single
+// solves): each k-row is split into at least CUs / N1 chunks, so that the
+// workgroups cover the CUs, down to 4 waves (4 bands on 120 x 120: 8 -> 4
+// waves, FP32 22.4 -> 20.7 ms, FP64 30.2 -> 26.8 ms).
 // lean: the lean FP32 form (k_sep_lean, KIND_SEPL; no LDS staging, at most
 // 168 VGPRs) of 5-8 bands with up to 12 waves per workgroup (one workgroup,
 // 3 waves per SIMD, per CU). For 3-4 bands it measured slower (120 x 120:
 // 23.1 ms staged, 29.4 ms lean with 4 waves), so they keep the staged form.
-static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = false) {
+static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = false, bool fill = false) {
     const SolveDesc& d = ds->d;
     LaunchKey k;
     k.fp32 = fp32;
@@ -2346,8 +2429,12 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = fal
     const size_t nstage = (fp32 && nbp <= 8 && !lean) ? (size_t)d.ns * d.nb * d.nb : 0;
     const int wmax = lean ? SEP_LEAN_MAX : 8;
     if (w <= 0) {
-        const int chunks = (d.N2 + wmax * gpw - 1) / (wmax * gpw);
+        int chunks = (d.N2 + wmax * gpw - 1) / (wmax * gpw);
         w = (d.N2 + chunks * gpw - 1) / (chunks * gpw);
+        if (fill && w > 4 && chunks < device_cus() / d.N1) {
+            chunks = device_cus() / d.N1;
+            w = std::max(4, (d.N2 + chunks * gpw - 1) / (chunks * gpw));
+        }
         while (w > 1 && sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd, d.ns).bytes > LDS_MAX) --w;
     }
     if (w < 1 || w > (lean ? SEP_LEAN_MAX : 8) || sep_layout(nbp, maxd, fp32 ? 4 : 8, w, nstage, d.nwd, d.ns).bytes > LDS_MAX)
@@ -2372,7 +2459,7 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
     if ((k = key_thread(ds, fp32)).kind) return k;
     if (d.nb > 64) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 64)");
     if (batch && (k = key_sep(ds, fp32, 0, true)).kind) return k;
-    if ((k = key_sep(ds, fp32, 0)).kind) {
+    if ((k = key_sep(ds, fp32, 0, false, !batch)).kind) {
         // a single solve whose lane groups carry one k-point each takes the
         // lean form when it applies (8 bands, 200 x 200: 191 -> 150 ms)
         LaunchKey kw;
@@ -2989,7 +3076,9 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         const double kpt = k.b * (64.0 / ((P / 2) * (P / 2))) * kpw;
         const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);
         const double table = 90.0 * d.ns * d.nb * d.nb;
-        return 8.0 * (3 * D * P2 + 4 * P2 * P) + 26.0 * P2 + D * (P2 / 4) * (P == 4 ? 120.0 : 32.0 + 96.0 / kpw) + table / (kpt * used);
+        // the 4-band FP32 matrix-core form forms H X - X H: two products per stage
+        const double prods = sep_mf(k.a, precision ? 4 : 8) ? 8.0 : 4.0;
+        return 8.0 * (3 * D * P2 + prods * P2 * P) + 26.0 * P2 + D * (P2 / 4) * (P == 4 ? 120.0 : 32.0 + 96.0 / kpw) + table / (kpt * used);
     }
     const double nb = d.nb;
     // element kernel: one nb^3 product per stage for Hermitian models, two otherwise
