@@ -51,7 +51,7 @@ def test_fp32_hmcase0_readme():
     assert err <= J_TOL and worst <= HARMONIC_TOL
 
 
-@pytest.mark.parametrize("nb,nr", [(2, 24), (4, 24), (8, 24), (16, 24), (20, 12)])
+@pytest.mark.parametrize("nb,nr", [(2, 24), (3, 24), (4, 24), (8, 24), (16, 24), (20, 12)])
 def test_fp32_synthetic(tmp_path, nb, nr):
     from hhgamd import tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
