@@ -1589,24 +1589,28 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
                 // and lane 4b + j receives D[r][j], r = 0..3 (its own column). With
                 // H, X Hermitian, C_ij = sum_m conj(H_mi) X_mj - conj(X_mi) H_mj, so
                 // both operands are the lanes' own column elements (m = 0..3):
-                //   Re C = sum_m (Hr_i Xr_j + Hi_i Xi_j) - (Xr_i Hr_j + Xi_i Hi_j)
-                //   Im C = sum_m (Hr_i Xi_j + Xi_i Hr_j) - (Hi_i Xr_j + Xr_i Hi_j)
-                // four accumulators of 8 rank-1 updates each (32 MFMAs per stage).
-                f32x4 rp = {0, 0, 0, 0}, rn = {0, 0, 0, 0}, ip1 = {0, 0, 0, 0}, in1 = {0, 0, 0, 0};
+                //   Re C = P - P^T,                    P  = sum_m Hr_i Xr_j + Hi_i Xi_j
+                //   Im C = (Q - S) + (Q - S)^T,        Q  = sum_m Hr_i Xi_j,  S = sum_m Hi_i Xr_j
+                // Each transpose is its own accumulator fed the same products in the
+                // same order (P^T = sum_m Xr_i Hr_j + Xi_i Hi_j), so Re C is exactly
+                // antisymmetric, Im C exactly symmetric and K exactly Hermitian, as in
+                // the M - M^dagger form (32 MFMAs per stage for the wave's 16 k-points).
+                f32x4 p = {0, 0, 0, 0}, pt = {0, 0, 0, 0}, qq = {0, 0, 0, 0}, qt = {0, 0, 0, 0},
+                      ss = {0, 0, 0, 0}, st = {0, 0, 0, 0};
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const float hr = Hs[m].re, hi = Hs[m].im, xr = rs[m].re, xi = rs[m].im;
-                    rp = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xr, rp, 0, 0, 0);
-                    rn = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hr, rn, 0, 0, 0);
-                    ip1 = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xi, ip1, 0, 0, 0);
-                    in1 = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xr, in1, 0, 0, 0);
-                    rp = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xi, rp, 0, 0, 0);
-                    rn = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hi, rn, 0, 0, 0);
-                    ip1 = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hr, ip1, 0, 0, 0);
-                    in1 = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hi, in1, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xr, p, 0, 0, 0);
+                    pt = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hr, pt, 0, 0, 0);
+                    qq = __builtin_amdgcn_mfma_f32_4x4x1f32(hr, xi, qq, 0, 0, 0);
+                    qt = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hr, qt, 0, 0, 0);
+                    ss = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xr, ss, 0, 0, 0);
+                    st = __builtin_amdgcn_mfma_f32_4x4x1f32(xr, hi, st, 0, 0, 0);
+                    p = __builtin_amdgcn_mfma_f32_4x4x1f32(hi, xi, p, 0, 0, 0);
+                    pt = __builtin_amdgcn_mfma_f32_4x4x1f32(xi, hi, pt, 0, 0, 0);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; ++q) K[q] = {ip1[q] - in1[q], rn[q] - rp[q]};
+                for (int q = 0; q < 4; ++q) K[q] = {(qq[q] - ss[q]) + (qt[q] - st[q]), pt[q] - p[q]};
                 return;
             } else {
             TT::st2(bufH, i0, j0, Hs[0], Hs[1]);

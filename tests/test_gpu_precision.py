@@ -71,3 +71,31 @@ def test_fp32_synthetic_16_bands_7x7_shell(tmp_path):
         assert s.kernel() == 6 and s.launch()["c"] == 8
     err, worst = check(prm)
     assert err <= J_TOL and worst <= HARMONIC_TOL
+
+
+@pytest.mark.parametrize("nb", [3, 4, 8])
+def test_fp32_state_stays_a_hermitian_projector(tmp_path, nb):
+    """After a full FP32-stage run each rho_k is exactly Hermitian (both the
+    M - M^dagger form and, for 3-4 bands, the matrix-core HX - XH form build
+    exactly Hermitian stage derivatives), keeps its initial trace to FP32
+    accuracy, stays as close to a projector as the FP64 path's RK4 state, and
+    agrees with that state. A commutator whose rounding breaks the symmetry
+    (Hermiticity ~1e-7, trace drift ~3e-5 here) fails the first two."""
+    from hhgamd import lib, tbgen
+    path = str(tmp_path / f"syn{nb}_tb.dat")
+    tbgen.write_tb(path, *tbgen.synthetic(nb, seed=70 + nb))
+    prm = make_param(path, nr=24, intensity=5e11, pol=(0.6, 0.8, 0.0))
+    states = {}
+    for prec in ("fp64", "fp32"):
+        with lib.Solve(dict(prm, precision=prec)) as s:
+            rho0 = s.rho_k()
+            s.run()
+            states[prec] = s.rho_k()
+    idem = {k: np.abs(np.einsum("kij,kjl->kil", r, r) - r).max() for k, r in states.items()}
+    rho = states["fp32"]
+    herm = np.abs(rho - np.conj(np.transpose(rho, (0, 2, 1)))).max()
+    dtr = np.abs(np.trace(rho, axis1=1, axis2=2) - np.trace(rho0, axis1=1, axis2=2)).max()
+    d64 = np.abs(rho - states["fp64"]).max()
+    print(f"nb={nb}: |rho - rho^H| {herm:.1e}, |dTr| {dtr:.1e}, |rho^2 - rho| {idem['fp32']:.1e} "
+          f"(FP64 {idem['fp64']:.1e}), |rho32 - rho64| {d64:.1e}")
+    assert herm < 1e-12 and dtr < 1e-5 and idem["fp32"] < idem["fp64"] + 1e-5 and d64 < 1e-5
