@@ -73,7 +73,7 @@ def test_fp32_synthetic_16_bands_7x7_shell(tmp_path):
     assert err <= J_TOL and worst <= HARMONIC_TOL
 
 
-@pytest.mark.parametrize("nb", [3, 4, 8])
+@pytest.mark.parametrize("nb", [2, 3, 4, 8, 16])
 def test_fp32_state_stays_a_hermitian_projector(tmp_path, nb):
     """After a full FP32-stage run each rho_k is exactly Hermitian (both the
     M - M^dagger form and, for 3-4 bands, the matrix-core HX - XH form build
@@ -84,7 +84,7 @@ def test_fp32_state_stays_a_hermitian_projector(tmp_path, nb):
     from hhgamd import lib, tbgen
     path = str(tmp_path / f"syn{nb}_tb.dat")
     tbgen.write_tb(path, *tbgen.synthetic(nb, seed=70 + nb))
-    prm = make_param(path, nr=24, intensity=5e11, pol=(0.6, 0.8, 0.0))
+    prm = make_param(path, nr=16 if nb > 8 else 24, intensity=5e11, pol=(0.6, 0.8, 0.0))
     states = {}
     for prec in ("fp64", "fp32"):
         with lib.Solve(dict(prm, precision=prec)) as s:
@@ -99,3 +99,7 @@ def test_fp32_state_stays_a_hermitian_projector(tmp_path, nb):
     print(f"nb={nb}: |rho - rho^H| {herm:.1e}, |dTr| {dtr:.1e}, |rho^2 - rho| {idem['fp32']:.1e} "
           f"(FP64 {idem['fp64']:.1e}), |rho32 - rho64| {d64:.1e}")
     assert herm < 1e-12 and dtr < 1e-5 and idem["fp32"] < idem["fp64"] + 1e-5 and d64 < 1e-5
+    # the FP64 path: exactly Hermitian, trace to rounding
+    r64 = states["fp64"]
+    assert np.abs(r64 - np.conj(np.transpose(r64, (0, 2, 1)))).max() < 1e-14
+    assert np.abs(np.trace(r64, axis1=1, axis2=2) - np.trace(rho0, axis1=1, axis2=2)).max() < 1e-11
