@@ -1196,7 +1196,7 @@ __host__ __device__ inline SepLayout sep_layout(int NBP, int MAXD, int tsize, in
     const int nbuf = sep_nbuf(NBP);
     const bool mf = sep_mf(NBP, tsize);
     size_t o = 0;
-    L.gbytes = 3 * (size_t)MAXD * E * 2 * tsize;
+    L.gbytes = 2 * (size_t)MAXD * E * 2 * tsize;  // fields E(t_i) and E(t_i+1)
     L.off_g = o; o = align16(o + nbuf * L.gbytes);
     L.gvbytes = (size_t)(NBP == 16 ? nwd : MAXD) * 3 * E * 16;
     L.off_gv = o; o = align16(o + nbuf * L.gvbytes);
@@ -1470,15 +1470,18 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
 #pragma unroll
                 for (int o = 0; o < NOPS; ++o) cfma(a[o], w, ob[o]);
             }
+            // stage fields E(t_i) and E(t_i+1) (slots 0, 1); the midpoint stages'
+            // H(E2) = (H(E1) + H(E3)) / 2 is formed per k-point, as E2 = (E1 + E3) / 2
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
+            for (int jj = 0; jj < 2; ++jj) {
+                const int j = 2 * jj;
                 cx<double> b = a[OP_H];
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
                     b.re = fmaT(ef[3 * j + x], a[OP_X + x].re, b.re);
                     b.im = fmaT(ef[3 * j + x], a[OP_X + x].im, b.im);
                 }
-                G[(size_t)(j * MAXD + dd) * E + TT::off(ei, ej)] = cast<T>(b);
+                G[(size_t)(jj * MAXD + dd) * E + TT::off(ei, ej)] = cast<T>(b);
             }
             if ((wmask >> dd) & 1) {
                 const int wr = wrank(dd);
@@ -1681,7 +1684,7 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             wave_lds_sync();
             }
         };
-        hstage(0);
+        hstage(0);  // H(E1)
 #pragma unroll
         for (int q = 0; q < 4; ++q) rs[q] = cast<T>(rho[q]);
         deriv();
@@ -1690,7 +1693,18 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             acc[q] = K[q];
             rs[q] = {fmaT(half, K[q].re, (T)rho[q].re), fmaT(half, K[q].im, (T)rho[q].im)};
         }
+        // H(E3), and the midpoint stages' H(E2) = (H(E1) + H(E3)) / 2 (the
+        // stage Hamiltonian is affine in the field and E2 = E1 / 2 + E3 / 2)
+        cx<T> H3[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) H3[q] = Hs[q];
         hstage(1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const cx<T> h1 = H3[q];
+            H3[q] = Hs[q];
+            Hs[q] = {(T)0.5 * (h1.re + Hs[q].re), (T)0.5 * (h1.im + Hs[q].im)};
+        }
         deriv();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -1703,7 +1717,8 @@ __device__ __forceinline__ void sep_body(const SolveDesc& d, const InstPtrs& ip,
             acc[q].re = fmaT((T)2, K[q].re, acc[q].re); acc[q].im = fmaT((T)2, K[q].im, acc[q].im);
             rs[q] = {fmaT(full, K[q].re, (T)rho[q].re), fmaT(full, K[q].im, (T)rho[q].im)};
         }
-        hstage(2);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Hs[q] = H3[q];
         deriv();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -3069,9 +3084,9 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     int nv = 0, nx = 0;
     for (int a = 0; a < 3; ++a) { nv += (d.vmask >> a) & 1; nx += (d.xmask >> a) & 1; }
     if (kind == KIND_SEP || kind == KIND_SEPL) {
-        // padded size P, table depth D (MAXD): stage sums 3 D P^2 cMAC,
-        // products 4 P^3 cMAC, the M - M^dagger combination and RK4 axpys
-        // ~26 P^2 flops, current per 2x2 lane block and value d 4 cMAC for the
+        // padded size P, table depth D (MAXD): stage sums 2 D P^2 cMAC (the
+        // midpoint stages average the two), products 4 P^3 cMAC, the
+        // M - M^dagger combination, the average and RK4 axpys ~30 P^2 flops, current per 2x2 lane block and value d 4 cMAC for the
         // twiddled state and 3 x 4 cMAC shared by the KPW k-points of the
         // group (3 (4 + 1) cMAC per k-point for 4 bands); plus the per-row tables (90 flops per support and element)
         // amortised over the k-points of a workgroup
@@ -3082,7 +3097,7 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         const double table = 90.0 * d.ns * d.nb * d.nb;
         // the 4-band FP32 matrix-core form forms H X - X H: two products per stage
         const double prods = sep_mf(k.a, precision ? 4 : 8) ? 8.0 : 4.0;
-        return 8.0 * (3 * D * P2 + prods * P2 * P) + 26.0 * P2 + D * (P2 / 4) * (P == 4 ? 120.0 : 32.0 + 96.0 / kpw) + table / (kpt * used);
+        return 8.0 * (2 * D * P2 + prods * P2 * P) + 30.0 * P2 + D * (P2 / 4) * (P == 4 ? 120.0 : 32.0 + 96.0 / kpw) + table / (kpt * used);
     }
     const double nb = d.nb;
     // element kernel: one nb^3 product per stage for Hermitian models, two otherwise

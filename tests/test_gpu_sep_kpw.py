@@ -18,12 +18,13 @@ FP64_TOL = 1e-10   # vs the oracle
 FP32_TOL = 1e-5    # J of FP32 stages vs the FP64 oracle
 SAME_TOL = 1e-13   # batch member vs the same solve alone
 
-# (bands, grid, precision) whose workgroups outnumber the CUs. 16 bands: rows
-# of 6 and 22 (FP64: 5 waves per workgroup) and 40 (FP32: 8 waves) k-points
-# end in partial tiles; 12 bands padded to 16 likewise; 8 and 6 (padded to 8)
+# (bands, grid, precision) whose workgroups outnumber the CUs. 16 bands (8
+# waves per workgroup): rows of 22 carry 4 k-points per wave in tiles of 8 x 4
+# (the second partly past the row's end), rows of 20 carry 2 in tiles of 7 x 2,
+# rows of 40 carry 2; 12 bands padded to 16 likewise; 8 and 6 (padded to 8)
 # bands: 5 waves of 4 k-points, rows of 36
-CASES = [(16, (130, 6), "fp64"), (16, (70, 22), "fp64"), (16, (80, 40), "fp64"), (16, (80, 40), "fp32"),
-         (12, (130, 6), "fp64"), (12, (80, 40), "fp32"),
+CASES = [(16, (200, 22), "fp64"), (16, (100, 20), "fp64"), (16, (80, 40), "fp64"), (16, (80, 40), "fp32"),
+         (12, (200, 22), "fp64"), (12, (80, 40), "fp32"),
          (8, (160, 36), "fp64"), (8, (160, 36), "fp32"), (6, (160, 36), "fp64")]
 
 
