@@ -113,6 +113,9 @@ int get_autotune();
 // Launch key the next run uses: kind, a, b, c, fp32.
 void dev_launch_key(DeviceSolve* ds, int precision, int* out5);
 int dev_kpoints_per_wave(DeviceSolve* ds, int precision);
+// Launch rules of a solve whose launches share the device with other solves
+// (the server's requests): those of batch members.
+void dev_set_shared(DeviceSolve* ds, bool shared);
 // Flops per k-point and RK4 step of the formulation the next run launches.
 double dev_flops_per_kstep(DeviceSolve* ds, int precision);
 // Arithmetic per k-point and RK4 step fixed by the model (Hermitian folding,

@@ -2238,6 +2238,9 @@ struct DeviceSolve {
     // 2 bands: every k-point has the same trace 2 r0 (the Bloch-vector kernel
     // applies); jr0[v] = r0 (m0 + m1) of the origin window block of v slot v
     bool r0_uniform = false;
+    // the solve's launches share the device with other solves (served
+    // requests): it takes the launch rules of batch members
+    bool shared = false;
     double jr0[3] = {0, 0, 0};
     int tkey[2][4] = {};
     float last_kernel_ms = 0.f;
@@ -2465,8 +2468,8 @@ static LaunchKey key_sep(const DeviceSolve* ds, int fp32, int w, bool lean = fal
     return k;
 }
 
-// batch: the solve runs in a batch, where solves of up to 8 bands keep one
-// k-point per lane group; 5-8-band FP32 members then take the lean form.
+// batch: the solve runs in a batch, where 5-8-band FP32 members take the lean
+// form (and carry two k-points per lane group on rows long enough, sep_kpw).
 static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch = false);
 static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool batch = false) {
     const SolveDesc& d = ds->d;
@@ -2505,7 +2508,7 @@ static LaunchKey classify(const DeviceSolve* ds, int precision, bool batch = fal
         k.kind = ds->tkey[p][0]; k.a = ds->tkey[p][1]; k.b = ds->tkey[p][2]; k.c = ds->tkey[p][3]; k.fp32 = p;
         return k;
     }
-    return classify_default(ds, precision, batch);
+    return classify_default(ds, precision, batch || ds->shared);
 }
 
 static int comps_of(const LaunchKey& k) {
@@ -2546,11 +2549,15 @@ static int device_cus() {
 // per step one table build (measured: about 1.2 k-point steps at 16 bands,
 // 0.3 at 8, where it overlaps the step) plus KPW k-point steps; the KPW of
 // 1, 2, 4 with the least total wins. In a batch the workgroups of many
-// instances keep every CU busy, so the 8-band gain from fewer lifetimes is
-// absent while the longer ones lengthen the batch's tail (measured 4 % slower
-// on a 64-instance mix): batched solves of up to 8 bands keep one k-point.
+// instances keep every CU busy and lifetimes do not count: a batched 5-8-band
+// solve (staged or lean form) carries two k-points per lane group whenever its
+// rows hold two tiles' worth, which halves the table builds and shares the
+// current's pass over the velocity tables (64-instance mix 3.23e9 -> 3.69e9
+// k-point steps/s). A single solve of the lean form keeps one.
 static int sep_kpw(const SolveDesc& d, const LaunchKey& k, bool batch) {
-    if (k.kind != KIND_SEP || sep_kpw_max(k.a) == 1 || (batch && k.a <= 8)) return 1;
+    if ((k.kind != KIND_SEP && k.kind != KIND_SEPL) || sep_kpw_max(k.a) == 1) return 1;
+    if (batch && k.a <= 8) return d.N2 >= 2 * k.b * (64 / ((k.a / 2) * (k.a / 2))) ? 2 : 1;
+    if (k.kind == KIND_SEPL) return 1;
     const int cus = device_cus();
     const int kpt1 = k.b * (64 / ((k.a / 2) * (k.a / 2)));
     const double build = k.a == 16 ? 1.2 : 0.3;
@@ -2702,11 +2709,11 @@ static int launch_with(DeviceSolve* ds, const LaunchKey& k, const SolveDesc& d, 
                        double* jpart = nullptr) {
     if (k.kind == KIND_THREAD && d.ns > k.b) throw std::runtime_error("thread-per-k kernel: support exceeds MAXU");
     int wpt = 0, block = 0;
-    const int ntiles = tiles_of(d, k, wpt, block);
+    const int ntiles = tiles_of(d, k, wpt, block, ds->shared);
     nwaves = ntiles * wpt;
     const int C = comps_of(k);
     if (!jpart) ensure_jpart(ds, nwaves, C);
-    InstPtrs ip = ptrs_of(ds, nwaves, sep_kpw(d, k));
+    InstPtrs ip = ptrs_of(ds, nwaves, sep_kpw(d, k, ds->shared));
     if (jpart) ip.jpart = jpart;
     const size_t lds = lds_of(d, k);
     dispatch(k, [&](auto single, auto) {
@@ -2816,7 +2823,8 @@ void dev_launch_key(DeviceSolve* ds, int precision, int* out5) {
     out5[0] = k.kind; out5[1] = k.a; out5[2] = k.b; out5[3] = k.c; out5[4] = k.fp32;
 }
 
-int dev_kpoints_per_wave(DeviceSolve* ds, int precision) { return sep_kpw(ds->d, classify(ds, precision)); }
+int dev_kpoints_per_wave(DeviceSolve* ds, int precision) { return sep_kpw(ds->d, classify(ds, precision), ds->shared); }
+void dev_set_shared(DeviceSolve* ds, bool shared) { ds->shared = shared; }
 
 void dev_launch(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
@@ -2865,7 +2873,7 @@ double dev_run_static(DeviceSolve* ds, int precision, double* J_out) {
     const SolveDesc& d = ds->d;
     const LaunchKey k = classify(ds, precision);
     int wpt = 0, block = 0;
-    const size_t parts = (size_t)tiles_of(d, k, wpt, block) * wpt;
+    const size_t parts = (size_t)tiles_of(d, k, wpt, block, ds->shared) * wpt;
     const size_t need = (size_t)comps_of(k) * d.nt * parts;
     if (need > STATIC_SCRATCH_DOUBLES)
         throw std::runtime_error("allocation-free run: the current partials need " + std::to_string(need * 8) +
@@ -3091,7 +3099,7 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         // group (3 (4 + 1) cMAC per k-point for 4 bands); plus the per-row tables (90 flops per support and element)
         // amortised over the k-points of a workgroup
         const LaunchKey k = classify(ds, precision);
-        const double P = k.a, P2 = P * P, D = k.c, kpw = sep_kpw(d, k);
+        const double P = k.a, P2 = P * P, D = k.c, kpw = sep_kpw(d, k, ds->shared);
         const double kpt = k.b * (64.0 / ((P / 2) * (P / 2))) * kpw;
         const double used = d.N2 / (std::ceil(d.N2 / kpt) * kpt);
         const double table = 90.0 * d.ns * d.nb * d.nb;

@@ -163,6 +163,7 @@ struct Server::Impl {
             for (auto& r : admit) {
                 try {
                     r->dev = dev_create(*r->host);
+                    dev_set_shared(r->dev, true);  // runs beside other requests: batch-member launch rules
                     dev_launch(r->dev, r->precision);
                 } catch (const std::exception& e) {
                     std::lock_guard<std::mutex> lk(mu);
