@@ -185,6 +185,7 @@ def main():
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per rate")
     ap.add_argument("--check-requests", type=int, default=24,
                     help="serve: requests of the top rate also solved one by one on the plain FP64 path")
+    ap.add_argument("--max-inflight", type=int, default=12, help="serve: solves on the device at once")
     ap.add_argument("--policy", default="least-work", choices=["least-work", "fifo"],
                     help="serve: admission order among prepared requests")
     args = ap.parse_args()
@@ -527,7 +528,7 @@ def bench_serve(args):
     rates = [float(r) for r in args.rates.split(",")]
     sweep = []
     recs = []
-    with serve.Server(hbm_budget_gb=128.0, max_inflight=8, host_threads=8, policy=args.policy) as srv:
+    with serve.Server(hbm_budget_gb=128.0, max_inflight=args.max_inflight, host_threads=8, policy=args.policy) as srv:
         warm = workloads.c5(tmp, 2, rate=100.0, seed=99, precision=args.precision)
         serve.run_stream(srv, [(0.0, p) for _, p in warm])
         for ri, rate in enumerate(rates):
