@@ -183,8 +183,9 @@ def main():
                     help="batch/serve/precision: tune every solve shape's launch configuration on its first run")
     ap.add_argument("--rates", default="20", help="serve: arrival rates (requests/s) to sweep")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per rate")
-    ap.add_argument("--check-requests", type=int, default=24,
-                    help="serve: requests of the top rate also solved one by one on the plain FP64 path")
+    ap.add_argument("--check-requests", type=int, default=None,
+                    help="serve: requests of the top rate also solved one by one on the plain FP64 path "
+                         "(default: all of them, so the served and the plain rates cover the same requests)")
     ap.add_argument("--max-inflight", type=int, default=12, help="serve: solves on the device at once")
     ap.add_argument("--policy", default="least-work", choices=["least-work", "fifo"],
                     help="serve: admission order among prepared requests")
@@ -518,10 +519,10 @@ def bench_serve(args):
     (default: --requests 200 at 20/s, a sustained stream past saturation);
     reports p50/p99 latency (arrival to J available) and k-point*RK4-steps/s.
     Every rate replays the same requests (one seed; the exponential gaps scale
-    with 1/rate). The first --check-requests requests of the top rate are also
-    solved one by one on the plain FP64 path: their throughput is the
-    reference point of the served rate and their currents check the served
-    FP32 results. The C4 instance mix is also streamed through the server at
+    with 1/rate). The requests of the top rate (the first --check-requests
+    when given) are also solved one by one on the plain FP64 path: their
+    throughput is the reference point of the served rate and their currents
+    check the served FP32 results. The C4 instance mix is also streamed through the server at
     100 requests/s, the like-for-like comparison with the batched rate."""
     from hhgamd import lib, serve, workloads
     tmp = tempfile.mkdtemp(prefix="hhg_bench_")
@@ -543,7 +544,9 @@ def bench_serve(args):
     top = len(rates) - 1
     plain_s, plain_ksteps = 0.0, 0
     errs = []
-    checked = workloads.c5(tmp, args.requests, rate=rates[top], seed=50, precision="fp64")[:args.check_requests]
+    checked = workloads.c5(tmp, args.requests, rate=rates[top], seed=50, precision="fp64")
+    if args.check_requests is not None:
+        checked = checked[:args.check_requests]
     for rec, (_, p) in zip(recs, checked):
         with lib.Solve(p) as s:
             a = time.perf_counter()
@@ -562,7 +565,7 @@ def bench_serve(args):
                        "rates_per_s": rates, "policy": args.policy},
             "latency_ms": {"p50": best["p50_latency_s"] * 1e3, "p99": best["p99_latency_s"] * 1e3},
             "plain_fp64_one_by_one_value": plain,
-            "plain_fp64_sample": f"the first {len(checked)} requests of the top rate, one by one",
+            "plain_fp64_sample": f"{len(checked)} of the {args.requests} requests of the top rate, one by one",
             "vs_plain_fp64_at_top_rate": sweep[top]["ksteps_per_s"] / plain if plain else None,
             "served_vs_fp64": {"results": len(errs),
                                "max_j_rel_l2": max((e[0] for e in errs), default=None),
