@@ -1154,15 +1154,18 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t
 // Row-separable element kernel: Hermitian H and rbc, 3 <= nb <= NBP (4, 8 or
 // 16, smaller nb zero-padded). A workgroup of W waves owns a chunk of one k-row
 // m1. Per step it builds, cooperatively and double-buffered in LDS,
-//   G[j][d]  = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} (H_s + E_j . rbc_s)   (stage fields j = 0..2)
+//   G[j][d]  = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} (H_s + E_j . rbc_s)   (stage fields E(t_i), E(t_i+1))
 //   GV[d][a] = sum_{s: n2_s = n2_d} tw1[m1 n1_s] e^{iA.R_s} v_a,s                 (axes a = x, y, z)
 // (for NBP = 16 GV holds only the values d with a velocity block, by rank)
 // and every k-point forms its stage Hamiltonian sum_d tw2[m2 n2_d] G[j][d]
-// with MAXD complex MACs per element instead of ns. For Hermitian H and X the
-// commutator is -i(M - M^dagger) with M = H X: one complex product per stage.
-// Each lane owns a 2x2 element block, so every LDS read of the product feeds
-// two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and exchange H,
-// X and M through LDS with wave-level ordering only. Each lane group carries
+// with MAXD complex MACs per element instead of ns (the midpoint stages'
+// field is the mean of the two, and so is their Hamiltonian). For Hermitian H
+// and X the commutator is -i(M - M^dagger) with M = H X: one complex product
+// per stage. Each lane owns a 2x2 element block, so every LDS read of the
+// product feeds two MACs; the (NBP/2)^2 lanes of a k-point sit in one wave and
+// exchange H, X and M through LDS with wave-level ordering only (4 bands in
+// FP32: a column per lane and the commutator from registers on the matrix
+// cores, below). Each lane group carries
 // KPW = 1, 2 or 4 k-points of the row (W groups apart) through every step, so
 // one build serves W * GPW * KPW k-points; their states rotate through one
 // register slot and the current reads GV once for all of them. One workgroup
