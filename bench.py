@@ -89,10 +89,11 @@ def cpu_baseline(path_tb, nr, sample_steps):
                       f"{dt:.1f} s"}
 
 
-def c1_full(path_tb):
+def c1_full(path_tb, threads):
     """The CPU-runnable configuration in full (30x30, 4096 RK4 steps): the GPU
-    plain path and the oracle's FFT-form restatement of the reference loop on
-    one core, both over the whole solve, and their J(t) agreement."""
+    plain path, the oracle's FFT-form restatement of the reference loop on one
+    core and its per-k form on `threads` host cores, each over the whole solve,
+    and their J(t) agreement."""
     import oracle
     from hhgamd import lib
     prm = workload_param(path_tb, 30, "fp64")
@@ -106,9 +107,14 @@ def c1_full(path_tb):
     a = time.perf_counter()
     _, Jc = oracle.run(path_tb, prm, mode="fft")
     cpu_s = time.perf_counter() - a
+    a = time.perf_counter()
+    _, Jp = oracle.run(path_tb, prm, mode="perk", nthreads=threads)
+    perk_s = time.perf_counter() - a
     return {"workload": "haldane-2band-30x30-nt4096-fp64", "gpu_value": nk * nt / gpu_s, "gpu_ms": gpu_s * 1e3,
             "cpu_fftform_value": nk * nt / cpu_s, "cpu_seconds": round(cpu_s, 2), "cpu_cores": 1,
-            "j_rel_l2_gpu_vs_cpu": float(np.linalg.norm(Jg - Jc) / np.linalg.norm(Jc))}
+            "cpu_perk_value": nk * nt / perk_s, "cpu_perk_seconds": round(perk_s, 2), "cpu_perk_cores": threads,
+            "j_rel_l2_gpu_vs_cpu": float(np.linalg.norm(Jg - Jc) / np.linalg.norm(Jc)),
+            "j_rel_l2_gpu_vs_cpu_perk": float(np.linalg.norm(Jg - Jp) / np.linalg.norm(Jp))}
 
 
 def cpu_perk_threads(path_tb, nr, steps, threads):
@@ -288,7 +294,7 @@ def bench_plain(args):
     if not args.no_cpu:  # CPU leg: the oracle, as baseline and as the FP64 checker of J(t)
         cores = cpu_cores()
         out["cpu_baseline"] = cpu_baseline(path_tb, args.nr, args.cpu_steps)
-        out["c1_full"] = c1_full(path_tb)
+        out["c1_full"] = c1_full(path_tb, cores["used"])
         out["cpu_perk_threads"] = cpu_perk_threads(path_tb, args.nr, 512, cores["used"])
         out["cpu_perk_threads"].update(cores)
         if not args.no_check:
@@ -344,6 +350,18 @@ def bench_precision(args):
         fp64_kms = s64.kernel_ms()
         roof64 = roofline_of(s64, fp64_kms, nk, nt, "fp64", s64.kernel(), 25)
     err, worst = j_errors(t, J32, J64, prm["lambda"])
+    cpu = None
+    if not args.no_cpu:  # the oracle's per-k FP64 form on the host cores, a bounded sample of steps
+        import oracle
+        cores = cpu_cores()
+        steps = 64
+        p64 = dict(prm, precision="fp64", tmax=prm["dt"] * steps + 1e-9)
+        a = time.perf_counter()
+        oracle.run(prm["path_tb"], p64, mode="perk", nthreads=cores["used"])
+        cs = time.perf_counter() - a
+        cpu = {"value": nk * oracle.nt(p64) / cs, "cores": cores["used"], "kind": "port",
+               "sample": f"oracle per-k FP64 form, {oracle.nt(p64)} RK4 steps of the same solve, {cs:.1f} s",
+               **cores}
     workload = f"synthetic-{nb}band-{prm['nr1']}x{prm['nr2']}-nt{nt}-fp32"
     roof["traffic"] = traffic_of(workload)
     return {"metric": METRIC, "value": nk * nt * args.steps / wall, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -354,7 +372,7 @@ def bench_precision(args):
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
             "j_rel_l2_vs_fp64": err, "worst_harmonic_dev_vs_fp64": worst,
             "fp64_value": nk * nt / fp64_s, "fp64_kernel_ms": fp64_kms, "fp32_vs_fp64": fp64_s / (wall / args.steps),
-            "roofline": roof, "roofline_fp64": roof64}
+            "cpu_perk_threads": cpu, "roofline": roof, "roofline_fp64": roof64}
 
 
 def bench_fftform(args):
