@@ -493,10 +493,10 @@ def bench_batch(args):
     for s in solves:
         s.run()
     seq = time.perf_counter() - t1
-    kinds = {}
+    kinds = {}  # launch configurations of the batch members: band count / kernel / waves / k-points per group
     for s, p in zip(solves, params):
-        L = s.launch()
-        key = f"nb{s.nb}/k{L['kernel']}" + (f"/w{L['b']}" if L["kernel"] in (6, 9) else "")
+        L = s.batch_launch()
+        key = f"nb{s.nb}/k{L['kernel']}" + (f"/w{L['b']}/kpw{L['kpw']}" if L["kernel"] in (6, 9) else "")
         kinds[key] = kinds.get(key, 0) + 1
     # plain FP64 path, one solve at a time, same instances; J(t) of each for the error check
     errs, worsts = [], []

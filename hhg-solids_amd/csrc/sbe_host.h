@@ -113,6 +113,8 @@ int get_autotune();
 // Launch key the next run uses: kind, a, b, c, fp32.
 void dev_launch_key(DeviceSolve* ds, int precision, int* out5);
 int dev_kpoints_per_wave(DeviceSolve* ds, int precision);
+// Launch key and k-points per lane group under the batch-member rules: out6.
+void dev_batch_launch_key(DeviceSolve* ds, int precision, int* out6);
 // Launch rules of a solve whose launches share the device with other solves
 // (the server's requests): those of batch members.
 void dev_set_shared(DeviceSolve* ds, bool shared);

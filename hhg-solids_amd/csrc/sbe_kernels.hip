@@ -2827,6 +2827,11 @@ void dev_launch_key(DeviceSolve* ds, int precision, int* out5) {
 }
 
 int dev_kpoints_per_wave(DeviceSolve* ds, int precision) { return sep_kpw(ds->d, classify(ds, precision), ds->shared); }
+void dev_batch_launch_key(DeviceSolve* ds, int precision, int* out6) {
+    const LaunchKey k = classify(ds, precision, true);
+    out6[0] = k.kind; out6[1] = k.a; out6[2] = k.b; out6[3] = k.c; out6[4] = k.fp32;
+    out6[5] = sep_kpw(ds->d, k, true);
+}
 void dev_set_shared(DeviceSolve* ds, bool shared) { ds->shared = shared; }
 
 void dev_launch(DeviceSolve* ds, int precision) {

@@ -354,6 +354,14 @@ API int SWESim_get_launch(SWESim* s, int* out5) {
     });
     return s ? s->status : -1;
 }
+API int SWESim_get_batch_launch(SWESim* s, int* out6) {
+    guarded(s, [&] {
+        if (!s->dev) throw std::runtime_error("SWESim_get_batch_launch: call SWESim_init first");
+        if (!out6) throw std::runtime_error("SWESim_get_batch_launch: null output");
+        hhg::dev_batch_launch_key(s->dev, s->precision, out6);
+    });
+    return s ? s->status : -1;
+}
 API int SWESim_get_kpoints_per_wave(SWESim* s) {
     int kpw = -1;
     guarded(s, [&] {

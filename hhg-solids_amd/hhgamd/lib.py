@@ -58,6 +58,7 @@ def bind(lib):
         "SWESim_autotune": ([V, I], I),
         "SWESim_get_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
         "SWESim_get_kpoints_per_wave": ([V], I),
+        "SWESim_get_batch_launch": ([V, np.ctypeslib.ndpointer(dtype=np.int32, flags="C")], I),
         "hhg_set_autotune": ([I], None), "hhg_get_autotune": ([], I),
         "hhg_fft2": ([np.ctypeslib.ndpointer(dtype=np.complex128, flags="C"), I, I, I, I], I),
         "hhg_fft2_plan_create": ([I, I, I], V), "hhg_fft2_plan_exec": ([V, V, I, I], Dd),
@@ -270,6 +271,14 @@ class Solve:
         self.lib.SWESim_get_launch(self.ptr, out)
         self._check("get_launch")
         return dict(zip(("kernel", "a", "b", "c", "fp32"), out.tolist()))
+
+    def batch_launch(self):
+        """Launch configuration as a batch member or served request: dict(kernel,
+        a, b, c, fp32, kpw)."""
+        out = np.zeros(6, dtype=np.int32)
+        self.lib.SWESim_get_batch_launch(self.ptr, out)
+        self._check("get_batch_launch")
+        return dict(zip(("kernel", "a", "b", "c", "fp32", "kpw"), out.tolist()))
 
     def kpoints_per_wave(self):
         """k-points per lane group of the next run on the row-separable kernel (1, 2 or 4; 1 elsewhere)."""

@@ -105,9 +105,13 @@ int SWESim_get_launch(SWESim* swe, int* out5);
 /* k-points each lane group of the next run carries through a step under one
  * per-row table build on the row-separable kernel: 1, 2 or 4 at 9-16 bands,
  * 1 or 2 at 5-8 bands (the value with the fewest workgroup lifetimes on this
- * device's CUs); 1 for every other kernel. Batched runs (hhg_batch_run) keep
- * 1 at up to 8 bands. Returns -1 on error. */
+ * device's CUs); 1 for every other kernel. Returns -1 on error. */
 int SWESim_get_kpoints_per_wave(SWESim* swe);
+/* The same two queries under the launch rules of a batch member (hhg_batch_run)
+ * or served request: out6 = {kernel, a, b, c, fp32, k-points per lane group}
+ * (5-8-band members carry 2 on rows of two tiles' worth). No counterpart in
+ * the reference. Returns 0 on success. */
+int SWESim_get_batch_launch(SWESim* swe, int* out6);
 /* Process-wide: when on, every solve (single, batched or served) is tuned on
  * its first run as by SWESim_autotune(swe, 32). Default off. */
 void hhg_set_autotune(int on);

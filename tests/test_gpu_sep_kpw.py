@@ -112,3 +112,33 @@ def test_lean_form_of_the_8_band_fp32_kernel(tmp_path, oracle_lib):
     for s, Ja in zip(solves, alone):
         assert rel_l2(s.current()[1], Ja) <= SAME_TOL
         s.close()
+
+
+def test_batched_5_to_8_bands_carry_two_kpoints_on_long_rows(tmp_path, oracle_lib):
+    """Batched 5-8-band members whose rows hold two tiles' worth carry two
+    k-points per lane group (staged FP64 form and lean FP32 form, last tiles
+    partly past the row end): each member matches its solo run to the
+    summation order of its current and the FP64 one matches the oracle."""
+    from hhgamd import lib
+    cases = [(8, (40, 80), "fp64"), (6, (40, 90), "fp64"), (8, (40, 110), "fp32"), (6, (40, 100), "fp32")]
+    params = [_param(_model(tmp_path, nb), nr, prec) for nb, nr, prec in cases]
+    alone = []
+    for p in params:
+        with lib.Solve(p) as s:
+            s.run()
+            alone.append(s.current()[1])
+    solves = [lib.Solve(p) for p in params]
+    for (nb, nr, prec), s in zip(cases, solves):
+        L = s.batch_launch()
+        assert L["kernel"] == (9 if prec == "fp32" else 6) and L["kpw"] == 2, (nb, nr, prec, L)
+    with lib.Batch(solves) as b:
+        b.run()
+    for (nb, nr, prec), p, s, Ja in zip(cases, params, solves, alone):
+        Jb = s.current()[1]
+        err = rel_l2(Jb, Ja)
+        print(f"nb={nb} nr={nr} {prec}: batch vs alone {err:.1e}")
+        assert err <= SAME_TOL
+        if prec == "fp64":
+            _, Jref = oracle_lib.run(p["path_tb"], p, mode="perk")
+            assert rel_l2(Jb, Jref) <= FP64_TOL
+        s.close()
