@@ -137,10 +137,23 @@ struct JStage {
 #pragma unroll
         for (int c = 0; c < C; ++c) lds[lane * ROW + slot * C + c] = v[c];
     }
-    // steps t0 .. t0+nsteps-1 are parked; jpart is [nt][nwaves][C]
+    // steps t0 .. t0+nsteps-1 are parked; jpart is [nt][nwaves][C].
+    // WG: the workgroup is this one wave and synchronises with barriers; else
+    // the LDS row block belongs to one wave of a larger workgroup, which
+    // orders its own LDS accesses (in issue order per wave) with wave fences.
+    template <bool WG = true>
     __device__ static void flush(const double* lds, int lane, int t0, int nsteps, double* jpart, int nwaves,
                                  int wave) {
-        __syncthreads();
+        auto sync = [] {
+            if constexpr (WG) {
+                __syncthreads();
+            } else {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        };
+        sync();
         if (lane < C * nsteps) {
             double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll 4
@@ -152,7 +165,7 @@ struct JStage {
             }
             jpart[((size_t)(t0 + lane / C) * nwaves + wave) * C + lane % C] = (a0 + a1) + (a2 + a3);
         }
-        __syncthreads();
+        sync();
     }
 };
 
@@ -548,11 +561,108 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
 // R != 0, added by the reduction -- and (m0 - m1) s_z + m2 s_x - m3 s_y.
 // Per k-point and step: 3 outputs per Fourier sum instead of 4, 6 VALU
 // operations per stage derivative instead of ~9, 3 state components.
+// Per pair and operator the 4 outputs (a, d, Re b, Im b) have coefficients
+// (c0, c1) of (Re w, Im w); with the twiddle w0 = e^{-i k.R} of the lane's
+// k-point folded in (w = w0 phi) and mapped to the Bloch components:
+//   g = (2 Re b, -2 Im b, a - d),   current u = (m2, -m3, m0 - m1).
+// g0, x0, u0: the origin terms (coefficients of the constant phase 1).
+template <int NREP, int NX, int NV, typename T>
+struct B2Coef {
+    static constexpr int NXA = NX > 0 ? NX : 1;
+    T hc[NREP][6], xc[NREP][NXA][6];
+    double vc[NREP][NV][6];
+    T g0[3], x0[NXA][3];
+    double u0[NV][3];
+    __device__ __forceinline__ void init(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int m1, int m2,
+                                         int lane) {
+        const cdptr coef = as_const(ip.pcoef);
+        const double lz = 0.0 * (double)lane;
+        auto fold = [](const double c0, const double c1, const cx<double> w0, double& a, double& b) {
+            a = c0 * w0.re + c1 * w0.im;
+            b = c1 * w0.re - c0 * w0.im;
+        };
+        // 4 outputs x 2 folded coefficients -> 3 Bloch components x 2
+        auto bloch = [&](const cdptr c, const cx<double> w0, const double sx, const double sy, double* out) {
+            double f[4][2];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fold(c[2 * j], c[2 * j + 1], w0, f[j][0], f[j][1]);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                out[u] = sx * f[2][u];
+                out[2 + u] = sy * f[3][u];
+                out[4 + u] = f[0][u] - f[1][u];
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
+            const cx<double> w0 = cmul(ld<double>(ip.tw1 + 2 * j1), ld<double>(ip.tw2 + 2 * j2));
+            const cdptr cr = coef + PAIR_HEAD + PAIR_REP * r;
+            double o[6];
+            bloch(cr, w0, 2.0, -2.0, o);
+#pragma unroll
+            for (int u = 0; u < 6; ++u) hc[r][u] = (T)o[u];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                bloch(cr + 8 + 8 * x, w0, 2.0, -2.0, o);
+#pragma unroll
+                for (int u = 0; u < 6; ++u) xc[r][x][u] = (T)o[u];
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) bloch(cr + 32 + 8 * v, w0, 1.0, -1.0, vc[r][v]);
+        }
+        g0[0] = (T)(2.0 * coef[2] + lz); g0[1] = (T)(-2.0 * coef[3] + lz); g0[2] = (T)(coef[0] - coef[1] + lz);
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+            const cdptr c = coef + 4 + 4 * x;
+            x0[x][0] = (T)(2.0 * c[2] + lz); x0[x][1] = (T)(-2.0 * c[3] + lz); x0[x][2] = (T)(c[0] - c[1] + lz);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const cdptr c = coef + 16 + 4 * v;
+            u0[v][0] = c[2] + lz; u0[v][1] = -c[3] + lz; u0[v][2] = c[0] - c[1] + lz;
+        }
+    }
+    // Fourier sums of one step row: g (gs) and the rbc vectors (xs)
+    __device__ __forceinline__ void sums(const double (&ph)[NREP][2], T (&gs)[3], T (&xs)[NXA][3]) const {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gs[j] = g0[j];
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const T wr = (T)ph[r][0], wi = (T)ph[r][1];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) gs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, gs[j]));
+        }
+#pragma unroll
+        for (int x = 0; x < NX; ++x) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[x][j] = x0[x][j];
+#pragma unroll
+            for (int r = 0; r < NREP; ++r) {
+                const T wr = (T)ph[r][0], wi = (T)ph[r][1];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, xs[x][j]));
+            }
+        }
+    }
+    // current of state s with the phases of one step (FP64), without the r0 term
+    __device__ __forceinline__ void current(const double (&ph)[NREP][2], double sx, double sy, double sz,
+                                            double (&jc)[3]) const {
+        jc[0] = jc[1] = jc[2] = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double u[3] = {u0[v][0], u0[v][1], u0[v][2]};
+#pragma unroll
+            for (int r = 0; r < NREP; ++r)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) u[j] = fmaT(vc[r][v][2 * j + 1], ph[r][1], fmaT(vc[r][v][2 * j], ph[r][0], u[j]));
+            jc[v] = fmaT(u[2], sz, fmaT(u[1], sy, u[0] * sx));
+        }
+    }
+};
+
 template <int NREP, int NX, int NV, typename T>
 __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
-    const cdptr coef = as_const(ip.pcoef);
-    const double* __restrict__ tw1 = ip.tw1;
-    const double* __restrict__ tw2 = ip.tw2;
     const double* __restrict__ rho_in = ip.rho_in;
     double* __restrict__ rho_out = ip.rho_out;
     double* __restrict__ jpart = ip.jpart;
@@ -569,61 +679,8 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
     __shared__ double jl[64 * JS::ROW];
     constexpr int NXA = NX > 0 ? NX : 1;
 
-    // Per pair and operator the 4 outputs (a, d, Re b, Im b) have coefficients
-    // (c0, c1) of (Re w, Im w); with the twiddle w0 = e^{-i k.R} folded in
-    // (w = w0 phi) and mapped to the Bloch components:
-    //   g = (2 Re b, -2 Im b, a - d),   current u = (m2, -m3, m0 - m1).
-    const double lz = 0.0 * (double)lane;
-    T hc[NREP][6], xc[NREP][NXA][6];
-    double vc[NREP][NV][6];
-    auto fold = [](const double c0, const double c1, const cx<double> w0, double& a, double& b) {
-        a = c0 * w0.re + c1 * w0.im;
-        b = c1 * w0.re - c0 * w0.im;
-    };
-    // 4 outputs x 2 folded coefficients -> 3 Bloch components x 2
-    auto bloch = [&](const cdptr c, const cx<double> w0, const double sx, const double sy, double* out) {
-        double f[4][2];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) fold(c[2 * j], c[2 * j + 1], w0, f[j][0], f[j][1]);
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            out[u] = sx * f[2][u];
-            out[2 + u] = sy * f[3][u];
-            out[4 + u] = f[0][u] - f[1][u];
-        }
-    };
-#pragma unroll
-    for (int r = 0; r < NREP; ++r) {
-        const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
-        const cx<double> w0 = cmul(ld<double>(tw1 + 2 * j1), ld<double>(tw2 + 2 * j2));
-        const cdptr cr = coef + PAIR_HEAD + PAIR_REP * r;
-        double o[6];
-        bloch(cr, w0, 2.0, -2.0, o);
-#pragma unroll
-        for (int u = 0; u < 6; ++u) hc[r][u] = (T)o[u];
-#pragma unroll
-        for (int x = 0; x < NX; ++x) {
-            bloch(cr + 8 + 8 * x, w0, 2.0, -2.0, o);
-#pragma unroll
-            for (int u = 0; u < 6; ++u) xc[r][x][u] = (T)o[u];
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) bloch(cr + 32 + 8 * v, w0, 1.0, -1.0, vc[r][v]);
-    }
-    // origin terms (coefficients of the constant phase 1)
-    T g0[3], x0[NXA][3];
-    double u0[NV][3];
-    g0[0] = (T)(2.0 * coef[2] + lz); g0[1] = (T)(-2.0 * coef[3] + lz); g0[2] = (T)(coef[0] - coef[1] + lz);
-#pragma unroll
-    for (int x = 0; x < NX; ++x) {
-        const cdptr c = coef + 4 + 4 * x;
-        x0[x][0] = (T)(2.0 * c[2] + lz); x0[x][1] = (T)(-2.0 * c[3] + lz); x0[x][2] = (T)(c[0] - c[1] + lz);
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-        const cdptr c = coef + 16 + 4 * v;
-        u0[v][0] = c[2] + lz; u0[v][1] = -c[3] + lz; u0[v][2] = c[0] - c[1] + lz;
-    }
+    B2Coef<NREP, NX, NV, T> cf;
+    cf.init(d, pd, ip, m1, m2, lane);
 
     double r0 = 0, sx = 0, sy = 0, sz = 0;
     if (active) {
@@ -645,27 +702,7 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
         double ph[NREP][2], ef[3][NXA];
         T gs[3], xs[NXA][3];
     };
-    auto sums = [&](Buf& b) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b.gs[j] = g0[j];
-#pragma unroll
-        for (int r = 0; r < NREP; ++r) {
-            const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) b.gs[j] = fmaT(hc[r][2 * j + 1], wi, fmaT(hc[r][2 * j], wr, b.gs[j]));
-        }
-#pragma unroll
-        for (int x = 0; x < NX; ++x) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) b.xs[x][j] = x0[x][j];
-#pragma unroll
-            for (int r = 0; r < NREP; ++r) {
-                const T wr = (T)b.ph[r][0], wi = (T)b.ph[r][1];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) b.xs[x][j] = fmaT(xc[r][x][2 * j + 1], wi, fmaT(xc[r][x][2 * j], wr, b.xs[x][j]));
-            }
-        }
-    };
+    auto sums = [&](Buf& b) { cf.sums(b.ph, b.gs, b.xs); };
     const int zl = lane * (int)(d.nt < 0);
     auto load_row = [&](int ti, Buf& b) {
         const double* row = ip.psteps + (size_t)ti * pd.stride + zl;
@@ -677,8 +714,9 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
             for (int x = 0; x < NXA; ++x) b.ef[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
     };
 
-    auto step = [&](int ti, const Buf& cur, Buf& nxt) {
-        if (ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+    // chk: the step reduces the previous chunk of the current when it starts one
+    auto step = [&](int ti, const Buf& cur, Buf& nxt, bool chk = true) {
+        if (chk && ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
         load_row(ti + 1 < d.nt ? ti + 1 : ti, nxt);
         T gx, gy, gz;
         auto stage_g = [&](int j) {
@@ -712,16 +750,8 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
         sz = fmaT(pref, (double)az, sz);
         // current of the new state with the phases of step ti (FP64), without
         // the r0 term; inactive lanes hold s = 0 and contribute exactly 0
-        double jc[3] = {0, 0, 0};
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            double u[3] = {u0[v][0], u0[v][1], u0[v][2]};
-#pragma unroll
-            for (int r = 0; r < NREP; ++r)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) u[j] = fmaT(vc[r][v][2 * j + 1], cur.ph[r][1], fmaT(vc[r][v][2 * j], cur.ph[r][0], u[j]));
-            jc[v] = fmaT(u[2], sz, fmaT(u[1], sy, u[0] * sx));
-        }
+        double jc[3];
+        cf.current(cur.ph, sx, sy, sz, jc);
         JS::put(jl, lane, ti % TCH, jc);
         sums(nxt);
     };
@@ -732,6 +762,16 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
     constexpr int CREGS = NREP * 6 * (1 + NX) * (int)(sizeof(T) / 4) + NREP * NV * 12;
     if constexpr (CREGS <= 160) {
         int ti = 0;
+        // whole chunks of TCH steps with no branch inside: the current of one
+        // step and the chain of the next are scheduled together
+        for (; ti + TCH <= d.nt; ti += TCH) {
+            if (ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+#pragma unroll
+            for (int u = 0; u < TCH; u += 2) {
+                step(ti + u, B0, B1, false);
+                step(ti + u + 1, B1, B0, false);
+            }
+        }
         for (; ti + 2 <= d.nt; ti += 2) {
             step(ti, B0, B1);
             step(ti + 1, B1, B0);
@@ -749,6 +789,177 @@ __device__ __forceinline__ void bloch2_body(const SolveDesc& d, const PairDesc& 
     }
     if (active) {
         double* o = rho_out + 2 * (size_t)k;
+        const size_t st = 2 * (size_t)d.N;
+        o[0] = r0 + sz; o[1] = 0.0;
+        o[st] = sx; o[st + 1] = -sy;
+        o[2 * st] = sx; o[2 * st + 1] = sy;
+        o[3 * st] = r0 - sz; o[3 * st + 1] = 0.0;
+    }
+}
+
+// Two-wave form of bloch2_body for grids of at most two groups of 64 k-points
+// per CU, where the one-wave form leaves SIMDs idle and its lone wave on a
+// SIMD exposes every latency of the step. Per group, wave 0 runs the step
+// (Fourier sums of the next row, the RK4 chain) and leaves each new state in
+// a ring of 2 SB LDS slots; wave 1 forms the current of those states one
+// block of SB steps later. One barrier per block; each wave passes
+// ceil(nt / SB) barriers. Every value is formed by the same operations in the
+// same order as in bloch2_body, so state and current are bitwise the same.
+template <int NREP, int NX, int NV, typename T>
+__device__ __forceinline__ void bloch2w_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
+    constexpr int NXA = NX > 0 ? NX : 1;
+    constexpr int TCH = 16;
+    constexpr int SB = 8;  // steps per barrier
+    using JS = JStage<3, TCH>;
+    __shared__ double jl[64 * JS::ROW];
+    __shared__ double sq[2 * SB][3][64];
+    const int nt = d.nt;
+    // Both waves run the steps in blocks of SB, unrolled (no branch inside a
+    // block), each followed by the barrier, and the rest of the steps (fewer
+    // than SB) followed by one more: ceil(nt / SB) barriers each. SB is even,
+    // so the two alternating step states line up across blocks.
+    auto run_steps = [&](auto&& stp) {
+        int i = 0;
+        for (; i + SB <= nt; i += SB) {
+#pragma unroll
+            for (int u = 0; u < SB; u += 2) {
+                stp(i + u, 0);
+                stp(i + u + 1, 1);
+            }
+            __syncthreads();
+        }
+        if (i < nt) {
+            for (; i + 2 <= nt; i += 2) {
+                stp(i, 0);
+                stp(i + 1, 1);
+            }
+            if (i < nt) stp(i, 0);
+            __syncthreads();
+        }
+    };
+    const int lane = threadIdx.x & 63;
+    const int k = bid * 64 + lane;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    // the role as a scalar: each wave takes exactly one branch, so both pass
+    // the same barriers
+    const int role = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    B2Coef<NREP, NX, NV, T> cf;
+    cf.init(d, pd, ip, m1, m2, lane);
+    // step rows are wave-uniform: scalar loads
+    auto load_ph = [&](int ti, double (&ph)[NREP][2]) {
+        const cdptr row = as_const(ip.psteps) + (size_t)ti * pd.stride;
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) { ph[r][0] = row[PAIR_STEP_HEAD + 2 * r]; ph[r][1] = row[PAIR_STEP_HEAD + 2 * r + 1]; }
+    };
+
+    if (role == 1) {  // wave 1: the current
+        // J(ti): the state after step ti (slot (ti + 1) % 2 SB) with the phases of row ti
+        auto current = [&](int ti, const double (&ph)[NREP][2]) {
+            const double* s = &sq[(ti + 1) % (2 * SB)][0][0];
+            double jc[3];
+            cf.current(ph, s[lane], s[64 + lane], s[128 + lane], jc);
+            JS::put(jl, lane, ti % TCH, jc);
+            if (ti % TCH == TCH - 1 && ti + 1 < nt)
+                JS::template flush<false>(jl, lane, ti + 1 - TCH, TCH, ip.jpart, ip.nwaves, bid);
+        };
+        // step i: J(i - SB), whose state wave 0 wrote in the block before;
+        // prefetches the phases of row i - SB + 1
+        auto fstep = [&](int i, const double (&pcur)[NREP][2], double (&pnext)[NREP][2]) {
+            if (i - SB + 1 >= 0) load_ph(i - SB + 1, pnext);
+            if (i - SB >= 0) current(i - SB, pcur);
+        };
+        double pj[2][NREP][2] = {};
+        run_steps([&](int i, int par) {
+            if (par == 0) fstep(i, pj[0], pj[1]);
+            else fstep(i, pj[1], pj[0]);
+        });
+        if (nt > 0) {
+            for (int t = nt > SB ? nt - SB : 0; t < nt; ++t) {  // the states of the last block
+                double ph[NREP][2];
+                load_ph(t, ph);
+                current(t, ph);
+            }
+            const int t0 = ((nt - 1) / TCH) * TCH;
+            JS::template flush<false>(jl, lane, t0, nt - t0, ip.jpart, ip.nwaves, bid);
+        }
+        return;
+    }
+
+    // wave 0: the step
+    double r0 = 0, sx = 0, sy = 0, sz = 0;
+    if (active) {
+        const double p = ip.rho_in[2 * (size_t)k];
+        const cx<double> c = ld<double>(ip.rho_in + 1 * 2 * (size_t)d.N + 2 * k);
+        const double q = ip.rho_in[3 * 2 * (size_t)d.N + 2 * k];
+        r0 = 0.5 * (p + q);
+        sz = 0.5 * (p - q);
+        sx = c.re;
+        sy = -c.im;
+    }
+    const T half = (T)(0.5 * ip.dt);
+    const T full = (T)ip.dt;
+    const double pref = 0.1666666666666 * ip.dt;
+    struct Buf {
+        double ph[NREP][2], ef[3][NXA];
+        T gs[3], xs[NXA][3];
+    };
+    auto load_row = [&](int ti, Buf& b) {
+        load_ph(ti, b.ph);
+        const cdptr row = as_const(ip.psteps) + (size_t)ti * pd.stride;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int x = 0; x < NXA; ++x) b.ef[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
+    };
+    auto cstep = [&](int ti, const Buf& cur, Buf& nxt) {
+        load_row(ti + 1 < nt ? ti + 1 : ti, nxt);
+        T gx, gy, gz;
+        auto stage_g = [&](int j) {
+            gx = cur.gs[0]; gy = cur.gs[1]; gz = cur.gs[2];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const T f = (T)cur.ef[j][x];
+                gx = fmaT(f, cur.xs[x][0], gx); gy = fmaT(f, cur.xs[x][1], gy); gz = fmaT(f, cur.xs[x][2], gz);
+            }
+        };
+        T kx, ky, kz;
+        auto deriv = [&](T ax, T ay, T az) {  // k = g x s
+            kx = fmaT(gy, az, -(gz * ay));
+            ky = fmaT(gz, ax, -(gx * az));
+            kz = fmaT(gx, ay, -(gy * ax));
+        };
+        const T Sx = (T)sx, Sy = (T)sy, Sz = (T)sz;
+        stage_g(0);
+        deriv(Sx, Sy, Sz);
+        T ax = kx, ay = ky, az = kz;
+        stage_g(1);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        stage_g(2);
+        deriv(fmaT(full, kx, Sx), fmaT(full, ky, Sy), fmaT(full, kz, Sz));
+        ax += kx; ay += ky; az += kz;
+        sx = fmaT(pref, (double)ax, sx);
+        sy = fmaT(pref, (double)ay, sy);
+        sz = fmaT(pref, (double)az, sz);
+        double* s = &sq[(ti + 1) % (2 * SB)][0][0];
+        s[lane] = sx;
+        s[64 + lane] = sy;
+        s[128 + lane] = sz;
+        cf.sums(nxt.ph, nxt.gs, nxt.xs);
+    };
+    Buf B0, B1;
+    load_row(0, B0);
+    cf.sums(B0.ph, B0.gs, B0.xs);
+    run_steps([&](int i, int par) {
+        if (par == 0) cstep(i, B0, B1);
+        else cstep(i, B1, B0);
+    });
+    if (active) {
+        double* o = ip.rho_out + 2 * (size_t)k;
         const size_t st = 2 * (size_t)d.N;
         o[0] = r0 + sz; o[1] = 0.0;
         o[st] = sx; o[st + 1] = -sy;
@@ -1871,6 +2082,10 @@ __global__ __launch_bounds__(64) void k_bloch2_batch(const DevInst* __restrict__
     bloch2_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
 }
 template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(128) void k_bloch2w(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
+    bloch2w_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
+}
+template <int NREP, int NX, int NV, typename T>
 __global__ __launch_bounds__(64) void k_pair2s(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
     pair2s_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
 }
@@ -2381,7 +2596,9 @@ size_t dev_bytes(const HostSolve& hs) {
 }
 
 // Which kernel and template instance a solve runs with.
-enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8, KIND_SEPL = 9 };
+enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8, KIND_SEPL = 9,
+       KIND_BLOCHW = 10 };
+static bool is_bloch(int kind) { return kind == KIND_BLOCH || kind == KIND_BLOCHW; }
 constexpr size_t LDS_MAX = 160 * 1024;
 struct LaunchKey {
     int kind = 0, a = 0, b = 0, c = 0, fp32 = 0;
@@ -2479,7 +2696,13 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
     const int fp32 = precision != 0;
     LaunchKey k;
     if (!reg_pairs_ok(ds->pd) && (k = key_pairs(ds, fp32)).kind) return k;
-    if ((k = key_bloch(ds, fp32)).kind) return k;
+    if ((k = key_bloch(ds, fp32)).kind) {
+        // a single solve whose two-wave workgroups (64 k-points each) leave a
+        // SIMD per wave takes the two-wave form: 30 x 30, 900 k-points, 1.01 ->
+        // 0.82 ms; at 200 x 200 (625 groups) the one-wave form is 2 % faster
+        if (!batch && (d.N + 63) / 64 <= 2 * device_cus()) k.kind = KIND_BLOCHW;
+        return k;
+    }
     if ((k = key_pair(ds, fp32)).kind) return k;
     if ((k = key_thread(ds, fp32)).kind) return k;
     if (d.nb > 64) throw std::runtime_error("unsupported band count " + std::to_string(d.nb) + " (at most 64)");
@@ -2509,27 +2732,28 @@ static LaunchKey classify(const DeviceSolve* ds, int precision, bool batch = fal
     if (ds->tuned[p]) {
         LaunchKey k;
         k.kind = ds->tkey[p][0]; k.a = ds->tkey[p][1]; k.b = ds->tkey[p][2]; k.c = ds->tkey[p][3]; k.fp32 = p;
+        if ((batch || ds->shared) && k.kind == KIND_BLOCHW) k.kind = KIND_BLOCH;  // no batched two-wave form
         return k;
     }
     return classify_default(ds, precision, batch || ds->shared);
 }
 
 static int comps_of(const LaunchKey& k) {
-    if (k.kind == KIND_PAIR || k.kind == KIND_BLOCH) return 3;
+    if (k.kind == KIND_PAIR || is_bloch(k.kind)) return 3;
     if (k.kind == KIND_PAIRS) return 2 * k.c;
     return 6;
 }
 
 static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
     OMap m;
-    const bool pair = k.kind == KIND_PAIR || k.kind == KIND_BLOCH;
+    const bool pair = k.kind == KIND_PAIR || is_bloch(k.kind);
     for (int c = 0; c < 6; ++c) {
         m.o[c] = (pair || k.kind == KIND_PAIRS) ? -1 : c;
         m.cst[c] = 0.0;
     }
     if (pair)
         for (int v = 0; v < ds->pd.nv; ++v) m.o[v] = 2 * ds->pd.vaxis[v];
-    if (k.kind == KIND_BLOCH)
+    if (is_bloch(k.kind))
         for (int v = 0; v < ds->pd.nv; ++v) m.cst[v] = ds->jr0[v];
     if (k.kind == KIND_PAIRS)
         for (int v = 0; v < ds->pd.nv; ++v) { m.o[2 * v] = 2 * ds->pd.vaxis[v]; m.o[2 * v + 1] = 2 * ds->pd.vaxis[v] + 1; }
@@ -2589,6 +2813,11 @@ static int tiles_of(const SolveDesc& d, const LaunchKey& k, int& waves_per_tile,
         const int kpb = block / G;
         return (d.N + kpb - 1) / kpb;
     }
+    if (k.kind == KIND_BLOCHW) {
+        block = 128;
+        waves_per_tile = 1;  // one current partial per group of 64 k-points and step
+        return (d.N + 63) / 64;
+    }
     block = 64;
     waves_per_tile = 1;
     return (d.N + 63) / 64;
@@ -2630,6 +2859,14 @@ static void dispatch(const LaunchKey& k, F&& f) {
                   k_bloch2_batch<R, X, V, float>);
             BLOCH(3, 0, 2) BLOCH(4, 0, 2) BLOCH(3, 0, 3) BLOCH(4, 0, 3) BLOCH(3, 2, 2)
 #undef BLOCH
+            break;
+        case KIND_BLOCHW:  // single solves only (batches take KIND_BLOCH)
+#define BLOCHW(R, X, V)                                                                                  \
+    if (k.a == R && k.b == X && k.c == V)                                                                \
+        return go(k_bloch2w<R, X, V, double>, k_bloch2_batch<R, X, V, double>, k_bloch2w<R, X, V, float>, \
+                  k_bloch2_batch<R, X, V, float>);
+            BLOCHW(3, 0, 2) BLOCHW(4, 0, 2) BLOCHW(3, 0, 3) BLOCHW(4, 0, 3) BLOCHW(3, 2, 2)
+#undef BLOCHW
             break;
         case KIND_PAIRS:
 #define PAIRS(R, X, V)                                                                                   \
@@ -2771,7 +3008,12 @@ static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
         for (int w = 1; w <= SEP_LEAN_MAX; ++w) add(key_sep(ds, fp32, w, true));
     }
     if (ds->pd.nrep > 0 && ds->use_herm) {
-        add(key_bloch(ds, fp32));
+        LaunchKey kb = key_bloch(ds, fp32);
+        add(kb);
+        if (kb.kind) {
+            kb.kind = KIND_BLOCHW;
+            add(kb);
+        }
         add(key_pair(ds, fp32));
         add(key_pairs(ds, fp32));
         add(key_thread(ds, fp32));
@@ -3084,7 +3326,7 @@ int dev_kernel_variant(DeviceSolve* ds, int precision) { return classify(ds, pre
 double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
     const SolveDesc& d = ds->d;
     const int kind = dev_kernel_variant(ds, precision);
-    if (kind == KIND_BLOCH) {
+    if (is_bloch(kind)) {
         // per pair 3 Bloch outputs x 2 FMAs for g and each rbc slot, stage
         // fields 18 nx, four cross products (4 x 9), stage inputs 18, RK4
         // accumulation 15, update 6, current nv (12 nrep + 5)

@@ -46,7 +46,7 @@ def test_autotuned_pair_kernels_match_oracle(oracle_lib):
         kind = s.autotune(steps=16)
         s.run()
         J = s.current()[1]
-    assert kind in (1, 2, 4, 8)
+    assert kind in (1, 2, 4, 8, 10)
     _, Jref = oracle_lib.run(HALDANE, p, mode="perk")
     assert rel_l2(J, Jref) <= 1e-10
 

@@ -46,8 +46,10 @@ def test_batch_members_equal_single_runs(tmp_path):
             alone.append(s.current()[1])
     solves = [lib.Solve(p) for p in params]
     kinds = sorted({s.kernel() for s in solves})
-    # register pairs (4-component and Bloch-vector), streamed pairs and row-separable kernels in one batch
-    assert kinds == [2, 4, 6, 8]
+    # register pairs (4-component and Bloch-vector), streamed pairs and row-separable kernels in one batch;
+    # alone, the Bloch-vector solves take the two-wave form (10), in the batch the one-wave form (8)
+    assert kinds == [2, 4, 6, 10]
+    assert sorted({s.batch_launch()["kernel"] for s in solves}) == [2, 4, 6, 8]
     with lib.Batch(solves) as b:
         b.run()
     for p, s, Ja in zip(params, solves, alone):
@@ -56,6 +58,8 @@ def test_batch_members_equal_single_runs(tmp_path):
         print(f"nr={p['nr1']}x{p['nr2']} kernel={s.kernel()} batch vs alone rel L2 = {err:.1e} "
               f"bitwise={np.array_equal(Jb, Ja)}")
         assert err <= SAME_TOL
+        if s.kernel() == 10:  # the two forms run the same operations in the same order
+            assert np.array_equal(Jb, Ja)
         s.close()
 
 
@@ -138,3 +142,51 @@ def test_member_changes_after_adding(tmp_path):
         b.close()
         A.close()
         B.close()
+
+
+@pytest.mark.parametrize("model,nr,nt", [
+    ("haldane", (160, 200), None),   # 500 groups of 64 k-points, 4096 steps: the largest two-wave grids
+    ("haldane", (30, 30), None),
+    ("haldane", (18, 22), 2), ("haldane", (18, 22), 3), ("haldane", (18, 22), 4),
+    ("haldane", (10, 6), 16), ("haldane", (10, 6), 17), ("haldane", (10, 6), 33),
+    ("hmcase0", (12, 12), 40), ("hmcase0", (14, 10), 47),
+])
+def test_two_wave_bloch_form_equals_one_wave_form(model, nr, nt):
+    """A Bloch-vector solve alone on a grid of at most two groups of 64
+    k-points per CU (512 groups on 256 CUs) runs the two-wave form (kind 10);
+    as the member of a batch it runs the one-wave form (kind 8). Both form every value
+    by the same operations in the same order: J(t) and the final state are
+    bitwise equal over two consecutive runs, for ragged last groups of 64
+    k-points, step counts around the 16-step current chunks and the two state
+    slots, and with rbc terms (hmcase0 at oblique polarisation)."""
+    from hhgamd import lib
+    path = {"haldane": HALDANE, "hmcase0": HMCASE0}[model]
+    kw = {} if nt is None else dict(tmax=0.02197 * nt + 1e-9, dt=0.02197, tmax_field=0.02197 * nt * 0.9,
+                                    intensity=4e12)
+    prm = make_param(path, nr=nr, pol=(0.6, 0.8, 0.0) if model == "hmcase0" else (0.0, 1.0, 0.0), **kw)
+    with lib.Solve(prm) as a, lib.Solve(prm) as b:
+        la, lb = a.launch(), b.batch_launch()
+        print(f"{model} {nr} nt={a.nt}: alone {la}, batched {lb}")
+        assert la["kernel"] == 10 and lb["kernel"] == 8 and (la["a"], la["b"], la["c"]) == (lb["a"], lb["b"], lb["c"])
+        if nt is not None:
+            assert a.nt == nt
+        with lib.Batch([b]) as bt:
+            for _ in range(2):
+                a.run()
+                bt.run()
+        steps = a.nt
+        Ja = np.stack([a.current_run(0)[1], a.current_run(1)[1]])
+        Jb = np.stack([b.current_run(0)[1], b.current_run(1)[1]])
+        ra, rb = a.rho_k(), b.rho_k()
+    assert Ja.shape == Jb.shape == (2, steps, 3)
+    assert np.array_equal(Ja, Jb) and np.array_equal(ra, rb)
+
+
+def test_bloch_form_by_grid_size():
+    """The 200 x 200 README-scale grid (625 groups) keeps the one-wave form
+    alone; 160 x 200 (500 groups) takes the two-wave form."""
+    from hhgamd import lib
+    with lib.Solve(make_param(HALDANE, nr=200)) as s:
+        assert s.kernel() == 8
+    with lib.Solve(make_param(HALDANE, nr=(160, 200))) as s:
+        assert s.kernel() == 10

@@ -137,7 +137,7 @@ def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model
         expect = (2, 1)
     else:
         path = {"hmcase0": HMCASE0, "haldane": HALDANE}[model]
-        expect = (8, 1)  # Bloch-vector kernel: one filled band at every k
+        expect = (10, 1)  # Bloch-vector kernel (two-wave form): one filled band at every k
     nr = (16, 20) if "8" not in model else (8, 10)
     prm = make_param(path, nr=nr, tmax=30.0 if "8" not in model else 8.0, tmax_field=25.0 if "8" not in model else 7.0,
                      intensity=3e12, pol=(0.6, 0.8, 0.0))
