@@ -461,10 +461,11 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
             for (int x = 0; x < NXA; ++x) b.ef[j][x] = NX > 0 ? row[3 * j + x] : 0.0;
     };
 
-    auto step = [&](int ti, const Buf& cur, Buf& nxt) {
-        // the previous chunk of TCH steps is reduced at the top of the step, so
-        // that the stage chain, the current and the next sums form one block
-        if (ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+    // chk: the step reduces the previous chunk of TCH steps when it starts a
+    // chunk (at its top, so that the stage chain, the current and the next
+    // sums form one block)
+    auto step = [&](int ti, const Buf& cur, Buf& nxt, bool chk = true) {
+        if (chk && ti % TCH == 0 && ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
         load_row(ti + 1 < d.nt ? ti + 1 : ti, nxt);
         T A, D, Br, Bi;
         auto stage_h = [&](int j) {
@@ -523,6 +524,15 @@ __device__ __forceinline__ void pair2_body(const SolveDesc& d, const PairDesc& p
     constexpr int CREGS = NREP * 8 * (1 + NX) * (int)(sizeof(T) / 4) + NREP * NV * 16;
     if constexpr (CREGS <= 160) {
         int ti = 0;
+        // whole chunks of TCH steps with no branch inside, as in bloch2_body
+        for (; ti + TCH <= d.nt; ti += TCH) {
+            if (ti > 0) JS::flush(jl, lane, ti - TCH, TCH, jpart, nwaves, wave);
+#pragma unroll
+            for (int u = 0; u < TCH; u += 2) {
+                step(ti + u, B0, B1, false);
+                step(ti + u + 1, B1, B0, false);
+            }
+        }
         for (; ti + 2 <= d.nt; ti += 2) {
             step(ti, B0, B1);
             step(ti + 1, B1, B0);
@@ -1016,7 +1026,8 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
     const T full = (T)dt;
     const double pref = 0.1666666666666 * dt;
 
-    for (int ti = 0; ti < d.nt; ++ti) {
+    // chk: the step reduces its chunk of the current when it ends one
+    auto step = [&](int ti, bool chk) {
         const cdptr row = steps + (size_t)ti * pd.stride;
         const cdptr cf = as_const(ip.scoef) + opaque_zero();
         const cfptr cf32 = (cfptr)(as_const(ip.scoef) + SC_N) + opaque_zero();
@@ -1099,8 +1110,19 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
             jc[2 * v + 1] = fmaT(mi[v][3], c.im, fmaT(mi[v][2], c.re, fmaT(mi[v][1], q, mi[v][0] * p)));
         }
         JS::put(jl, lane, ti % TCH, jc);
-        if (ti % TCH == TCH - 1 || ti == d.nt - 1) JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
+        if (chk && (ti % TCH == TCH - 1 || ti == d.nt - 1))
+            JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
+    };
+    // whole chunks of TCH steps with no branch inside (unrolled by two), then
+    // the rest step by step: the current of one step and the next step's
+    // sums and chain are scheduled together
+    int ti = 0;
+    for (; ti + TCH <= d.nt; ti += TCH) {
+#pragma unroll 2
+        for (int u = 0; u < TCH; ++u) step(ti + u, false);
+        JS::flush(jl, lane, ti, TCH, jpart, nwaves, wave);
     }
+    for (; ti < d.nt; ++ti) step(ti, true);
     if (active) {
         double* o = rho_out + 2 * (size_t)k;
         const size_t st = 2 * (size_t)d.N;
