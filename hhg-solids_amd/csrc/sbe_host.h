@@ -81,9 +81,13 @@ void write_test_files(const TBModel& m, const HostSolve& hs, const std::string& 
 // Device side (sbe_kernels.hip).
 struct DeviceSolve;
 DeviceSolve* dev_create(const HostSolve& hs);
-// Propagates nt steps from the current device state. J_out: [nt][6].
+// Propagates nt steps from the current device state. J_out: [nt][6], or
+// null to leave J(t) in the solve's pinned host buffer only (dev_host_current).
 // Returns the device time in milliseconds.
 double dev_run(DeviceSolve* ds, int precision, double* J_out);
+// J(t) of the last completed run, [nt][6], in the solve's pinned host buffer
+// (valid until the next run of the solve).
+const double* dev_host_current(const DeviceSolve* ds);
 // The same run without allocating device memory: the current partials go to
 // the library's static scratch array on the solve's device (at most
 // static_scratch_bytes(); larger launches are refused). Concurrent calls of

@@ -137,13 +137,17 @@ API void SWESim_run_simulation(SWESim* s) {
     guarded(s, [&] {
         if (!s->dev) throw std::runtime_error("SWESim_run_simulation: call SWESim_init first (SWESim_init_host sets up the host tables only)");
         const int nt = s->host->d.nt;
-        std::vector<double> J((size_t)6 * nt);
+        std::vector<double> Jf;
+        const double* J = nullptr;  // [nt][6]
         if (s->variant == 2) {  // the reference's real-space FFT algorithm on the GPU
             if (s->precision != 0) throw std::runtime_error("the FFT form runs in FP64 only");
             if (!s->ff) s->ff = hhg::fftform_create(*s->host);
-            s->last_ms = hhg::fftform_run(s->ff, J.data());
-        } else {
-            s->last_ms = hhg::dev_run(s->dev, s->precision, J.data());
+            Jf.resize((size_t)6 * nt);
+            s->last_ms = hhg::fftform_run(s->ff, Jf.data());
+            J = Jf.data();
+        } else {  // read straight from the solve's pinned host buffer
+            s->last_ms = hhg::dev_run(s->dev, s->precision, nullptr);
+            J = hhg::dev_host_current(s->dev);
         }
         for (int a = 0; a < 3; ++a)
             for (int ti = 0; ti < nt; ++ti) s->J[a].push_back(cplx(J[(size_t)6 * ti + 2 * a], J[(size_t)6 * ti + 2 * a + 1]));
