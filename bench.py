@@ -194,6 +194,10 @@ def main():
                     help="serve: requests of the top rate also solved one by one on the plain FP64 path "
                          "(default: all of them, so the served and the plain rates cover the same requests)")
     ap.add_argument("--max-inflight", type=int, default=12, help="serve: solves on the device at once")
+    ap.add_argument("--light-rate", type=float, default=2.0,
+                    help="serve: arrival rate of the light-load stream (latency below saturation)")
+    ap.add_argument("--light-requests", type=int, default=40,
+                    help="serve: requests of the light-load stream (0: none)")
     ap.add_argument("--policy", default="least-work", choices=["least-work", "fifo"],
                     help="serve: admission order among prepared requests")
     args = ap.parse_args()
@@ -535,7 +539,9 @@ def bench_batch(args):
 def bench_serve(args):
     """Poisson streams of mixed solves (nb in {2,4,8,16}, 5-smooth grids in
     [48, 240], tmax U[60, 150]) through the server at each arrival rate
-    (default: --requests 200 at 20/s, a sustained stream past saturation);
+    (default: --requests 200 at 20/s, a sustained stream past saturation),
+    then --light-requests of the same generator at --light-rate (latency
+    below saturation);
     reports p50/p99 latency (arrival to J available) and k-point*RK4-steps/s.
     Every rate replays the same requests (one seed; the exponential gaps scale
     with 1/rate). The requests of the top rate (the first --check-requests
@@ -558,6 +564,12 @@ def bench_serve(args):
             summ["rate_per_s"] = rate
             sweep.append(summ)
             print(json.dumps({"serve_rate": rate, **summ}), file=sys.stderr, flush=True)
+        light = None
+        if args.light_requests > 0:  # the same generator's first requests, spaced out
+            la = workloads.c5(tmp, args.light_requests, rate=args.light_rate, seed=50, precision=args.precision)
+            _, light = serve.run_stream(srv, la)
+            light["rate_per_s"] = args.light_rate
+            print(json.dumps({"serve_light_rate": args.light_rate, **light}), file=sys.stderr, flush=True)
         c4 = workloads.c4(tmp, n=64, precision=args.precision)
         _, c4summ = serve.run_stream(srv, [(i / 100.0, p) for i, p in enumerate(c4)])
     top = len(rates) - 1
@@ -589,6 +601,7 @@ def bench_serve(args):
             "served_vs_fp64": {"results": len(errs),
                                "max_j_rel_l2": max((e[0] for e in errs), default=None),
                                "worst_harmonic_dev": max((e[1] for e in errs), default=None)},
+            "light_load": light,
             "c4_mix_served": {"value": c4summ["ksteps_per_s"], "requests": c4summ["requests"],
                               "rate_per_s": 100.0, "p50_latency_s": c4summ["p50_latency_s"],
                               "p99_latency_s": c4summ["p99_latency_s"]},
