@@ -1113,12 +1113,13 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
         if (chk && (ti % TCH == TCH - 1 || ti == d.nt - 1))
             JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
     };
-    // whole chunks of TCH steps with no branch inside (unrolled by two), then
-    // the rest step by step: the current of one step and the next step's
-    // sums and chain are scheduled together
+    // whole chunks of TCH steps with no branch inside (unrolled by two up to
+    // 12 pairs; 32 pairs would spill), then the rest step by step: the current
+    // of one step and the next step's sums and chain are scheduled together
+    constexpr int UNR = NREP <= 12 ? 2 : 1;
     int ti = 0;
     for (; ti + TCH <= d.nt; ti += TCH) {
-#pragma unroll 2
+#pragma unroll UNR
         for (int u = 0; u < TCH; ++u) step(ti + u, false);
         JS::flush(jl, lane, ti, TCH, jpart, nwaves, wave);
     }
