@@ -144,26 +144,27 @@ def test_member_changes_after_adding(tmp_path):
         B.close()
 
 
-@pytest.mark.parametrize("model,nr,nt", [
-    ("haldane", (160, 200), None),   # 500 groups of 64 k-points, 4096 steps: the largest two-wave grids
-    ("haldane", (30, 30), None),
-    ("haldane", (18, 22), 2), ("haldane", (18, 22), 3), ("haldane", (18, 22), 4),
-    ("haldane", (10, 6), 16), ("haldane", (10, 6), 17), ("haldane", (10, 6), 33),
-    ("hmcase0", (12, 12), 40), ("hmcase0", (14, 10), 47),
+@pytest.mark.parametrize("model,nr,nt,prec", [
+    ("haldane", (160, 200), None, "fp64"),   # 500 groups of 64 k-points, 4096 steps: the largest two-wave grids
+    ("haldane", (30, 30), None, "fp64"), ("haldane", (30, 30), None, "fp32"),
+    ("haldane", (18, 22), 2, "fp64"), ("haldane", (18, 22), 3, "fp64"), ("haldane", (18, 22), 4, "fp64"),
+    ("haldane", (10, 6), 16, "fp64"), ("haldane", (10, 6), 17, "fp64"), ("haldane", (10, 6), 33, "fp64"),
+    ("hmcase0", (12, 12), 40, "fp64"), ("hmcase0", (14, 10), 47, "fp64"), ("hmcase0", (12, 12), 40, "fp32"),
 ])
-def test_two_wave_bloch_form_equals_one_wave_form(model, nr, nt):
+def test_two_wave_bloch_form_equals_one_wave_form(model, nr, nt, prec):
     """A Bloch-vector solve alone on a grid of at most two groups of 64
     k-points per CU (512 groups on 256 CUs) runs the two-wave form (kind 10);
     as the member of a batch it runs the one-wave form (kind 8). Both form every value
     by the same operations in the same order: J(t) and the final state are
     bitwise equal over two consecutive runs, for ragged last groups of 64
     k-points, step counts around the 16-step current chunks and the two state
-    slots, and with rbc terms (hmcase0 at oblique polarisation)."""
+    slots, with rbc terms (hmcase0 at oblique polarisation) and FP32 stages."""
     from hhgamd import lib
     path = {"haldane": HALDANE, "hmcase0": HMCASE0}[model]
     kw = {} if nt is None else dict(tmax=0.02197 * nt + 1e-9, dt=0.02197, tmax_field=0.02197 * nt * 0.9,
                                     intensity=4e12)
-    prm = make_param(path, nr=nr, pol=(0.6, 0.8, 0.0) if model == "hmcase0" else (0.0, 1.0, 0.0), **kw)
+    prm = dict(make_param(path, nr=nr, pol=(0.6, 0.8, 0.0) if model == "hmcase0" else (0.0, 1.0, 0.0), **kw),
+               precision=prec)
     with lib.Solve(prm) as a, lib.Solve(prm) as b:
         la, lb = a.launch(), b.batch_launch()
         print(f"{model} {nr} nt={a.nt}: alone {la}, batched {lb}")
