@@ -2658,8 +2658,13 @@ static LaunchKey key_pairs(const DeviceSolve* ds, int fp32) {  // streamed coeff
     if (!(pd.nrep > 0 && ds->use_herm)) return k;
     k.kind = KIND_PAIRS;
     k.a = pd.nrep <= 12 ? 12 : 32;
+    // rbc slots and current axes as the model and the polarisation need them
+    // (an in-plane field leaves two rbc axes live; a 3-D velocity, three axes)
+    const bool r12 = k.a == 12;
     if (pd.nx == 0 && pd.nv <= 2) { k.b = 0; k.c = 2; }
     else if (pd.nx <= 2 && pd.nv <= 2) { k.b = 2; k.c = 2; }
+    else if (r12 && pd.nx == 0 && pd.nv <= 3) { k.b = 0; k.c = 3; }
+    else if (r12 && pd.nx <= 2 && pd.nv <= 3) { k.b = 2; k.c = 3; }
     else { k.b = 3; k.c = 3; }
     return k;
 }
@@ -2896,7 +2901,8 @@ static void dispatch(const LaunchKey& k, F&& f) {
     if (k.a == R && k.b == X && k.c == V)                                                                \
         return go(k_pair2s<R, X, V, double>, k_pair2s_batch<R, X, V, double>, k_pair2s<R, X, V, float>,  \
                   k_pair2s_batch<R, X, V, float>);
-            PAIRS(12, 0, 2) PAIRS(12, 2, 2) PAIRS(12, 3, 3) PAIRS(32, 0, 2) PAIRS(32, 2, 2) PAIRS(32, 3, 3)
+            PAIRS(12, 0, 2) PAIRS(12, 2, 2) PAIRS(12, 0, 3) PAIRS(12, 2, 3) PAIRS(12, 3, 3)
+            PAIRS(32, 0, 2) PAIRS(32, 2, 2) PAIRS(32, 3, 3)
 #undef PAIRS
             break;
         case KIND_THREAD:
