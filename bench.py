@@ -36,7 +36,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X published vector peaks
 ATTAINABLE_FP32 = 78.6  # non-packed FP32 VALU: one FMA per lane and instruction, the FP64 rate
 PEAK_HBM_GBS = 8000.0
 KERNELS = {1: "k_thread_per_k", 2: "k_pair2", 3: "k_elem_per_lane", 4: "k_pair2s", 6: "k_sep", 8: "k_bloch2", 9: "k_sep_lean",
-           10: "k_bloch2w"}
+           10: "k_bloch2w", 11: "k_bloch2s"}
 
 
 def j_errors(t, J, Jref, lam):

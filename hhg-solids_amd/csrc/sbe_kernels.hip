@@ -72,6 +72,9 @@ __device__ __forceinline__ cdptr as_const(const double* p) { return (cdptr)(p); 
 // Streamed pair coefficients: SC_N doubles, then the same table as floats
 // (read by the FP32 kernels, which would otherwise convert every value).
 constexpr int SC_N = PS_HEAD + PAIR_MAX * PS_REP;
+// The streamed Bloch-vector pack follows in the same buffer: BS_N doubles at
+// BS_OFF, then the same table as floats.
+constexpr int BS_OFF = SC_N + (SC_N + 1) / 2;
 // An opaque zero: offsets a per-step load so it is not hoisted out of the loop.
 __device__ __forceinline__ int opaque_zero() {
     int z = 0;
@@ -1134,6 +1137,157 @@ __device__ __forceinline__ void pair2s_body(const SolveDesc& d, const PairDesc& 
     }
 }
 
+// nb = 2 in Bloch-vector form with the pair coefficients streamed (complex
+// window current): the streamed counterpart of bloch2_body for models whose
+// every k-point has the same occupation. Per pair and step: the twiddled phase
+// w, 3 Fourier-sum components for g and for each rbc slot, 6 for each current
+// axis, instead of 4 and 8 in pair2s_body; a stage is one cross product. The
+// r0 (m_0 + m_1) part of the current sums, over the grid, to the constant of
+// the origin, which the reduction adds.
+template <int NREP, int NX, int NV, typename T>
+__device__ __forceinline__ void bloch2s_body(const SolveDesc& d, const PairDesc& pd, const InstPtrs& ip, int bid) {
+    const cdptr steps = as_const(ip.psteps);
+    const double* __restrict__ rho_in = ip.rho_in;
+    double* __restrict__ rho_out = ip.rho_out;
+    double* __restrict__ jpart = ip.jpart;
+    const int nwaves = ip.nwaves;
+    const double dt = ip.dt;
+    const int k = bid * 64 + threadIdx.x;
+    const bool active = k < d.N;
+    const int kk = active ? k : 0;
+    const int m1 = kk / d.N2, m2 = kk - (kk / d.N2) * d.N2;
+    const int wave = bid;
+    const int lane = threadIdx.x & 63;
+    constexpr int C = 2 * NV;
+    constexpr int TCH = 64 / C;
+    using JS = JStage<C, TCH>;
+    __shared__ double jl[64 * JS::ROW];
+    constexpr int NXA = NX > 0 ? NX : 1;
+
+    cx<double> w0[NREP];
+#pragma unroll
+    for (int r = 0; r < NREP; ++r) {
+        w0[r] = {0.0, 0.0};
+        if (r < pd.nrep) {
+            const int j1 = posmod((long)m1 * pd.n1[r], d.N1), j2 = posmod((long)m2 * pd.n2[r], d.N2);
+            w0[r] = cmul(ld<double>(ip.tw1 + 2 * j1), ld<double>(ip.tw2 + 2 * j2));
+        }
+    }
+    double r0 = 0, sx = 0, sy = 0, sz = 0;
+    if (active) {
+        const double p = rho_in[2 * (size_t)k];
+        const cx<double> c = ld<double>(rho_in + 1 * 2 * (size_t)d.N + 2 * k);
+        const double q = rho_in[3 * 2 * (size_t)d.N + 2 * k];
+        r0 = 0.5 * (p + q);
+        sz = 0.5 * (p - q);
+        sx = c.re;
+        sy = -c.im;
+    }
+    const T half = (T)(0.5 * dt);
+    const T full = (T)dt;
+    const double pref = 0.1666666666666 * dt;
+
+    auto step = [&](int ti, bool chk) {
+        const cdptr row = steps + (size_t)ti * pd.stride;
+        const cdptr cf = as_const(ip.scoef) + BS_OFF + opaque_zero();
+        const cfptr cf32 = (cfptr)(as_const(ip.scoef) + BS_OFF + BS_N) + opaque_zero();
+        auto cT = [&](int i) -> T {  // g and rbc coefficient i in the stage type
+            if constexpr (std::is_same<T, float>::value) return cf32[i];
+            else return (T)cf[i];
+        };
+        cx<double> w[NREP];
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) w[r] = cmul(w0[r], ld<double>(row + PAIR_STEP_HEAD + 2 * r));
+        T gs[3], xs[NXA][3];
+        double ur[NV][3], ui[NV][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            gs[j] = cT(j);
+#pragma unroll
+            for (int x = 0; x < NX; ++x) xs[x][j] = cT(3 + 3 * x + j);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { ur[v][j] = cf[12 + 3 * v + j]; ui[v][j] = cf[21 + 3 * v + j]; }
+        }
+#pragma unroll
+        for (int r = 0; r < NREP; ++r) {
+            const int o = BS_HEAD + BS_REP * r;
+            const T wr = (T)w[r].re, wi = (T)w[r].im;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) gs[j] = fmaT(cT(o + 2 * j + 1), wi, fmaT(cT(o + 2 * j), wr, gs[j]));
+#pragma unroll
+            for (int x = 0; x < NX; ++x)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    xs[x][j] = fmaT(cT(o + 6 + 6 * x + 2 * j + 1), wi, fmaT(cT(o + 6 + 6 * x + 2 * j), wr, xs[x][j]));
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const cdptr cv = cf + o + 24 + 12 * v;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    ur[v][j] = fmaT(cv[2 * j + 1], w[r].im, fmaT(cv[2 * j], w[r].re, ur[v][j]));
+                    ui[v][j] = fmaT(cv[6 + 2 * j + 1], w[r].im, fmaT(cv[6 + 2 * j], w[r].re, ui[v][j]));
+                }
+            }
+        }
+        T gx, gy, gz;
+        auto stage_g = [&](int j) {
+            gx = gs[0]; gy = gs[1]; gz = gs[2];
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const T f = (T)row[3 * j + x];
+                gx = fmaT(f, xs[x][0], gx); gy = fmaT(f, xs[x][1], gy); gz = fmaT(f, xs[x][2], gz);
+            }
+        };
+        T kx, ky, kz;
+        auto deriv = [&](T ax, T ay, T az) {  // k = g x s
+            kx = fmaT(gy, az, -(gz * ay));
+            ky = fmaT(gz, ax, -(gx * az));
+            kz = fmaT(gx, ay, -(gy * ax));
+        };
+        const T Sx = (T)sx, Sy = (T)sy, Sz = (T)sz;
+        stage_g(0);
+        deriv(Sx, Sy, Sz);
+        T ax = kx, ay = ky, az = kz;
+        stage_g(1);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        deriv(fmaT(half, kx, Sx), fmaT(half, ky, Sy), fmaT(half, kz, Sz));
+        ax = fmaT((T)2, kx, ax); ay = fmaT((T)2, ky, ay); az = fmaT((T)2, kz, az);
+        stage_g(2);
+        deriv(fmaT(full, kx, Sx), fmaT(full, ky, Sy), fmaT(full, kz, Sz));
+        ax += kx; ay += ky; az += kz;
+        sx = fmaT(pref, (double)ax, sx);
+        sy = fmaT(pref, (double)ay, sy);
+        sz = fmaT(pref, (double)az, sz);
+        // current of the new state (FP64), without the r0 term; inactive lanes
+        // hold s = 0 and contribute exactly 0
+        double jc[C];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            jc[2 * v] = fmaT(ur[v][2], sz, fmaT(ur[v][1], sy, ur[v][0] * sx));
+            jc[2 * v + 1] = fmaT(ui[v][2], sz, fmaT(ui[v][1], sy, ui[v][0] * sx));
+        }
+        JS::put(jl, lane, ti % TCH, jc);
+        if (chk && (ti % TCH == TCH - 1 || ti == d.nt - 1))
+            JS::flush(jl, lane, ti - ti % TCH, ti % TCH + 1, jpart, nwaves, wave);
+    };
+    int ti = 0;
+    for (; ti + TCH <= d.nt; ti += TCH) {
+#pragma unroll 2
+        for (int u = 0; u < TCH; ++u) step(ti + u, false);
+        JS::flush(jl, lane, ti, TCH, jpart, nwaves, wave);
+    }
+    for (; ti < d.nt; ++ti) step(ti, true);
+    if (active) {
+        double* o = rho_out + 2 * (size_t)k;
+        const size_t st = 2 * (size_t)d.N;
+        o[0] = r0 + sz; o[1] = 0.0;
+        o[st] = sx; o[st + 1] = -sy;
+        o[2 * st] = sx; o[2 * st + 1] = sy;
+        o[3 * st] = r0 - sz; o[3 * st + 1] = 0.0;
+    }
+}
+
 // --------------------------------------------------------------------------
 // Element-per-lane kernel for any nb <= 64. A group of G lanes owns one k
 // (G = nb*nb rounded up to a power of two, at most 64, for nb <= 16; the whole
@@ -2109,6 +2263,15 @@ __global__ __launch_bounds__(128) void k_bloch2w(const SolveDesc d, const PairDe
     bloch2w_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
 }
 template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_bloch2s(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
+    bloch2s_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
+}
+template <int NREP, int NX, int NV, typename T>
+__global__ __launch_bounds__(64) void k_bloch2s_batch(const DevInst* __restrict__ insts, const int2* __restrict__ tiles) {
+    const int2 t = tiles[blockIdx.x];
+    bloch2s_body<NREP, NX, NV, T>(insts[t.x].d, insts[t.x].pd, insts[t.x].p, t.y);
+}
+template <int NREP, int NX, int NV, typename T>
 __global__ __launch_bounds__(64) void k_pair2s(const SolveDesc d, const PairDesc pd, const InstPtrs ip) {
     pair2s_body<NREP, NX, NV, T>(d, pd, ip, blockIdx.x);
 }
@@ -2483,6 +2646,7 @@ struct DeviceSolve {
     // requests): it takes the launch rules of batch members
     bool shared = false;
     double jr0[3] = {0, 0, 0};
+    double jr0s[3][2] = {};  // the same for the streamed Bloch-vector kernel: (Re, Im) of the complex current
     int tkey[2][4] = {};
     float last_kernel_ms = 0.f;
 };
@@ -2502,6 +2666,37 @@ static double* dalloc(size_t bytes, hipStream_t s) {
     double* p = nullptr;
     HIP_CHECK(hipMallocAsync((void**)&p, std::max<size_t>(bytes, 16), s));
     return p;
+}
+
+// Streamed Bloch-vector pack (layout in sbe_types.h) from the streamed pack.
+static void bloch_pack(const double* sc, const PairDesc& pd, double* b) {
+    std::fill(b, b + BS_N, 0.0);
+    // origin outputs (a, d, Re b, Im b) -> g; current (m_0..3) -> u
+    auto g3 = [](const double* h, double* o) { o[0] = 2 * h[2]; o[1] = -2 * h[3]; o[2] = h[0] - h[1]; };
+    auto u3 = [](const double* m, double* o) { o[0] = m[2]; o[1] = -m[3]; o[2] = m[0] - m[1]; };
+    g3(sc, b);
+    for (int x = 0; x < pd.nx; ++x) g3(sc + 4 + 4 * x, b + 3 + 3 * x);
+    for (int v = 0; v < pd.nv; ++v) {
+        u3(sc + 16 + 8 * v, b + 12 + 3 * v);
+        u3(sc + 16 + 8 * v + 4, b + 21 + 3 * v);
+    }
+    // pair coefficients: output j, coefficient u of (Re w, Im w) at c[2 j + u]
+    auto gp = [](const double* c, double* q) {
+        for (int u = 0; u < 2; ++u) { q[u] = 2 * c[4 + u]; q[2 + u] = -2 * c[6 + u]; q[4 + u] = c[u] - c[2 + u]; }
+    };
+    auto up = [](const double* c, double* q) {
+        for (int u = 0; u < 2; ++u) { q[u] = c[4 + u]; q[2 + u] = -c[6 + u]; q[4 + u] = c[u] - c[2 + u]; }
+    };
+    for (int r = 0; r < pd.nrep; ++r) {
+        const double* cr = sc + PS_HEAD + PS_REP * r;
+        double* o = b + BS_HEAD + BS_REP * r;
+        gp(cr, o);
+        for (int x = 0; x < pd.nx; ++x) gp(cr + 8 + 8 * x, o + 6 + 6 * x);
+        for (int v = 0; v < pd.nv; ++v) {
+            up(cr + 32 + 16 * v, o + 24 + 12 * v);          // Re J
+            up(cr + 32 + 16 * v + 8, o + 24 + 12 * v + 6);  // Im J
+        }
+    }
 }
 
 DeviceSolve* dev_create(const HostSolve& hs) {
@@ -2545,14 +2740,17 @@ DeviceSolve* dev_create(const HostSolve& hs) {
         if (hs.pd.nrep > 0) {
             if (!hs.pcoef.empty()) ds->pcoef = upload(hs.pcoef, ds->stream);
             if (hs.scoef.size() != (size_t)SC_N) throw std::runtime_error("pair coefficient table of unexpected size");
-            sc.assign(SC_N + (SC_N + 1) / 2, 0.0);
+            sc.assign(BS_OFF + BS_N + (BS_N + 1) / 2, 0.0);
             std::copy(hs.scoef.begin(), hs.scoef.end(), sc.begin());
             float* f = reinterpret_cast<float*>(sc.data() + SC_N);
             for (int i = 0; i < SC_N; ++i) f[i] = (float)hs.scoef[i];
+            bloch_pack(hs.scoef.data(), hs.pd, sc.data() + BS_OFF);
+            float* fb = reinterpret_cast<float*>(sc.data() + BS_OFF + BS_N);
+            for (int i = 0; i < BS_N; ++i) fb[i] = (float)sc[BS_OFF + i];
             ds->scoef = upload(sc, ds->stream);
             ds->psteps = upload(hs.psteps, ds->stream);
         }
-        if (hs.d.nb == 2 && hs.pd.nrep > 0 && hs.pd.vpaired && !hs.pcoef.empty()) {
+        if (hs.d.nb == 2 && hs.pd.nrep > 0) {
             // r0_k = (rho00 + rho11) / 2 of the initial state
             const size_t N = hs.d.N;
             std::vector<double> pq(4 * N);
@@ -2574,7 +2772,11 @@ DeviceSolve* dev_create(const HostSolve& hs) {
             }
             ds->r0_uniform = hi - lo <= 1e-12;
             const double r0 = sum / (double)N;
-            for (int v = 0; v < hs.pd.nv; ++v) ds->jr0[v] = r0 * (hs.pcoef[16 + 4 * v] + hs.pcoef[16 + 4 * v + 1]);
+            for (int v = 0; v < hs.pd.nv; ++v) {
+                if (!hs.pcoef.empty()) ds->jr0[v] = r0 * (hs.pcoef[16 + 4 * v] + hs.pcoef[16 + 4 * v + 1]);
+                ds->jr0s[v][0] = r0 * (hs.scoef[16 + 8 * v] + hs.scoef[16 + 8 * v + 1]);
+                ds->jr0s[v][1] = r0 * (hs.scoef[16 + 8 * v + 4] + hs.scoef[16 + 8 * v + 5]);
+            }
         }
         ds->rho_tmp = dalloc(sizeof(double) * 2 * (size_t)hs.d.nb * hs.d.nb * hs.d.N, ds->stream);
         ds->J = dalloc(sizeof(double) * 6 * (size_t)hs.d.nt, ds->stream);
@@ -2614,13 +2816,14 @@ size_t dev_bytes(const HostSolve& hs) {
     const size_t rho = 2 * (size_t)E * d.N;
     const size_t init = (hs.rho0.empty() && init_needs_scratch(d.nb)) ? 6 * (size_t)E * d.N : 0;
     return sizeof(double) * (hs.blk.size() + hs.steptab.size() + hs.tw1.size() + hs.tw2.size() + 2 * rho + init +
-                             hs.pcoef.size() + hs.scoef.size() * 3 / 2 + 1 + hs.psteps.size() + 6 * (size_t)d.nt +
+                             hs.pcoef.size() + (hs.scoef.empty() ? 0 : (size_t)(BS_OFF + BS_N + (BS_N + 1) / 2)) + 1 +
+                             hs.psteps.size() + 6 * (size_t)d.nt +
                              6 * (size_t)d.nt * waves);
 }
 
 // Which kernel and template instance a solve runs with.
 enum { KIND_THREAD = 1, KIND_PAIR = 2, KIND_ELEM = 3, KIND_PAIRS = 4, KIND_SEP = 6, KIND_BLOCH = 8, KIND_SEPL = 9,
-       KIND_BLOCHW = 10 };
+       KIND_BLOCHW = 10, KIND_BLOCHS = 11 };
 static bool is_bloch(int kind) { return kind == KIND_BLOCH || kind == KIND_BLOCHW; }
 constexpr size_t LDS_MAX = 160 * 1024;
 struct LaunchKey {
@@ -2666,6 +2869,13 @@ static LaunchKey key_pairs(const DeviceSolve* ds, int fp32) {  // streamed coeff
     else if (r12 && pd.nx == 0 && pd.nv <= 3) { k.b = 0; k.c = 3; }
     else if (r12 && pd.nx <= 2 && pd.nv <= 3) { k.b = 2; k.c = 3; }
     else { k.b = 3; k.c = 3; }
+    return k;
+}
+// Bloch-vector form of key_pairs (the same occupation at every k; 12-pair instances)
+static LaunchKey key_blochs(const DeviceSolve* ds, int fp32) {
+    LaunchKey k = key_pairs(ds, fp32);
+    if (k.kind && ds->r0_uniform && k.a == 12) k.kind = KIND_BLOCHS;
+    else k.kind = 0;
     return k;
 }
 static LaunchKey key_thread(const DeviceSolve* ds, int fp32) {
@@ -2723,6 +2933,7 @@ static LaunchKey classify_default(const DeviceSolve* ds, int precision, bool bat
     const SolveDesc& d = ds->d;
     const int fp32 = precision != 0;
     LaunchKey k;
+    if (!reg_pairs_ok(ds->pd) && (k = key_blochs(ds, fp32)).kind) return k;
     if (!reg_pairs_ok(ds->pd) && (k = key_pairs(ds, fp32)).kind) return k;
     if ((k = key_bloch(ds, fp32)).kind) {
         // a single solve whose two-wave workgroups (64 k-points each) leave a
@@ -2768,23 +2979,26 @@ static LaunchKey classify(const DeviceSolve* ds, int precision, bool batch = fal
 
 static int comps_of(const LaunchKey& k) {
     if (k.kind == KIND_PAIR || is_bloch(k.kind)) return 3;
-    if (k.kind == KIND_PAIRS) return 2 * k.c;
+    if (k.kind == KIND_PAIRS || k.kind == KIND_BLOCHS) return 2 * k.c;
     return 6;
 }
 
 static OMap omap_of(const DeviceSolve* ds, const LaunchKey& k) {
     OMap m;
     const bool pair = k.kind == KIND_PAIR || is_bloch(k.kind);
+    const bool streamed = k.kind == KIND_PAIRS || k.kind == KIND_BLOCHS;
     for (int c = 0; c < 6; ++c) {
-        m.o[c] = (pair || k.kind == KIND_PAIRS) ? -1 : c;
+        m.o[c] = (pair || streamed) ? -1 : c;
         m.cst[c] = 0.0;
     }
     if (pair)
         for (int v = 0; v < ds->pd.nv; ++v) m.o[v] = 2 * ds->pd.vaxis[v];
     if (is_bloch(k.kind))
         for (int v = 0; v < ds->pd.nv; ++v) m.cst[v] = ds->jr0[v];
-    if (k.kind == KIND_PAIRS)
+    if (streamed)
         for (int v = 0; v < ds->pd.nv; ++v) { m.o[2 * v] = 2 * ds->pd.vaxis[v]; m.o[2 * v + 1] = 2 * ds->pd.vaxis[v] + 1; }
+    if (k.kind == KIND_BLOCHS)
+        for (int v = 0; v < ds->pd.nv; ++v) { m.cst[2 * v] = ds->jr0s[v][0]; m.cst[2 * v + 1] = ds->jr0s[v][1]; }
     return m;
 }
 
@@ -2895,6 +3109,14 @@ static void dispatch(const LaunchKey& k, F&& f) {
                   k_bloch2_batch<R, X, V, float>);
             BLOCHW(3, 0, 2) BLOCHW(4, 0, 2) BLOCHW(3, 0, 3) BLOCHW(4, 0, 3) BLOCHW(3, 2, 2)
 #undef BLOCHW
+            break;
+        case KIND_BLOCHS:
+#define BLOCHS(R, X, V)                                                                                  \
+    if (k.a == R && k.b == X && k.c == V)                                                                \
+        return go(k_bloch2s<R, X, V, double>, k_bloch2s_batch<R, X, V, double>, k_bloch2s<R, X, V, float>, \
+                  k_bloch2s_batch<R, X, V, float>);
+            BLOCHS(12, 0, 2) BLOCHS(12, 2, 2) BLOCHS(12, 0, 3) BLOCHS(12, 2, 3) BLOCHS(12, 3, 3)
+#undef BLOCHS
             break;
         case KIND_PAIRS:
 #define PAIRS(R, X, V)                                                                                   \
@@ -3038,6 +3260,7 @@ static std::vector<LaunchKey> candidates(const DeviceSolve* ds, int precision) {
     }
     if (ds->pd.nrep > 0 && ds->use_herm) {
         LaunchKey kb = key_bloch(ds, fp32);
+        add(key_blochs(ds, fp32));
         add(kb);
         if (kb.kind) {
             kb.kind = KIND_BLOCHW;
@@ -3363,6 +3586,15 @@ double dev_flops_per_kstep(DeviceSolve* ds, int precision) {
         // accumulation 15, update 6, current nv (12 nrep + 5)
         const PairDesc& pd = ds->pd;
         return 12.0 * pd.nrep * (1 + pd.nx) + 18.0 * pd.nx + 36 + 18 + 15 + 6 + pd.nv * (12.0 * pd.nrep + 5);
+    }
+    if (kind == KIND_BLOCHS) {
+        // per pair the twiddled phase (6), 3 Bloch outputs x 2 FMAs for g and
+        // each rbc slot, 6 x 2 FMAs per current axis; stage fields 18 nx, four
+        // cross products 36, stage inputs 18, RK4 accumulation 15, update 6,
+        // current nv x 2 x 5
+        const PairDesc& pd = ds->pd;
+        return 6.0 * pd.nrep + 12.0 * pd.nrep * (1 + pd.nx) + 24.0 * pd.nrep * pd.nv + 18.0 * pd.nx + 36 + 18 + 15 +
+               6 + 10.0 * pd.nv;
     }
     if (kind == KIND_PAIR || kind == KIND_PAIRS) {
         const PairDesc& pd = ds->pd;

@@ -80,6 +80,18 @@ constexpr int PAIR_STEP_HEAD = 10;
 constexpr int PS_HEAD = 40;
 constexpr int PS_REP = 80;
 
+// Streamed Bloch-vector pack (k_bloch2s: 2 bands, the same occupation at every
+// k), derived from the streamed pack at upload. With rho = r0 I + s.sigma,
+// s = (Re c, -Im c, (p - q) / 2), a Hermitian output set (a, d, Re b, Im b)
+// maps to g = (2 Re b, -2 Im b, a - d) and a current (m_0..3) to
+// u = (m_2, -m_3, m_0 - m_1), J = u.s + r0 (m_0 + m_1). Layout (doubles):
+//   [0, 3) g0, [3, 12) rbc0 slots, [12, 21) u0 slots (Re J), [21, 30) u0' (Im J)
+//   BS_HEAD + BS_REP r: g (3 x 2), rbc slots (3 x 3 x 2), v slots (3 x (Re J 3 x 2, Im J 3 x 2))
+// Coefficient 2*j + {0: Re w, 1: Im w} of component j.
+constexpr int BS_HEAD = 32;
+constexpr int BS_REP = 60;
+constexpr int BS_N = BS_HEAD + PAIR_MAX * BS_REP;
+
 struct PairDesc {
     int nrep, nx, nv, stride;
     int vpaired;          // 1: every v block has its -R partner, the current is real

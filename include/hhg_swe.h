@@ -92,7 +92,9 @@ void SWESim_set_kernel_variant(SWESim* swe, int variant);
  * bands, batched or one k-point per lane group: no LDS staging of the blocks,
  * up to 12 waves per workgroup), 10 = the Bloch-vector form of 8 on two waves
  * per 64 k-points (single solves of at most two such groups per CU; batch
- * members and served requests take 8); 0 before SWESim_init. */
+ * members and served requests take 8), 11 = Hermitian nb=2 with streamed
+ * pairs in Bloch-vector form (every k-point has the same occupation, up to 12
+ * pairs, complex window current); 0 before SWESim_init. */
 int SWESim_get_kernel(SWESim* swe);
 /* Autotuner: times every launch configuration that covers this solve (kernel
  * family, workgroup size) on its first `steps` RK4 steps without changing

@@ -12,7 +12,7 @@ FP64_TOL = 1e-10     # vs the oracle
 
 
 def _mixed_params(tmp_path):
-    from hhgamd import tbgen
+    from hhgamd import tbgen, workloads
     files = {}
     for nb in (2, 3, 4, 8):
         files[nb] = str(tmp_path / f"syn{nb}_tb.dat")
@@ -33,6 +33,8 @@ def _mixed_params(tmp_path):
         make_param(files[8], nr=(6, 6), pol=(0.8, 0.6, 0.0), tmax=4.0, tmax_field=3.0, intensity=2e12),
         dict(make_param(files[4], nr=(6, 8), pol=(0.6, 0.8, 0.0), **short), precision="fp32"),
         make_param(doped, nr=(16, 20), pol=(0.6, 0.8, 0.0), **short),
+        # a filled lower band at every k-point: streamed pairs in Bloch-vector form
+        make_param(workloads.synthetic_file(str(tmp_path), 2), nr=(8, 12), pol=(0.6, 0.8, 0.0), **short),
     ]
 
 
@@ -46,10 +48,11 @@ def test_batch_members_equal_single_runs(tmp_path):
             alone.append(s.current()[1])
     solves = [lib.Solve(p) for p in params]
     kinds = sorted({s.kernel() for s in solves})
-    # register pairs (4-component and Bloch-vector), streamed pairs and row-separable kernels in one batch;
-    # alone, the Bloch-vector solves take the two-wave form (10), in the batch the one-wave form (8)
-    assert kinds == [2, 4, 6, 10]
-    assert sorted({s.batch_launch()["kernel"] for s in solves}) == [2, 4, 6, 8]
+    # register pairs (4-component and Bloch-vector), streamed pairs (4-component and Bloch-vector) and
+    # row-separable kernels in one batch; alone, the register Bloch-vector solves take the two-wave form (10), in the batch
+    # the one-wave form (8)
+    assert kinds == [2, 4, 6, 10, 11]
+    assert sorted({s.batch_launch()["kernel"] for s in solves}) == [2, 4, 6, 8, 11]
     with lib.Batch(solves) as b:
         b.run()
     for p, s, Ja in zip(params, solves, alone):

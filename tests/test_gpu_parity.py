@@ -114,18 +114,30 @@ def test_synthetic_bands_vs_oracle(native, oracle_lib, tmp_path, nb, nr, tmax, n
     assert np.abs(np.trace(rho, axis1=1, axis2=2) - tr0).max() < 1e-10
 
 
-@pytest.mark.parametrize("model", ["hmcase0", "haldane", "haldane_doped", "synthetic2", "synthetic4", "synthetic8"])
+@pytest.mark.parametrize("model", ["hmcase0", "haldane", "haldane_doped", "synthetic2", "synthetic2_filled",
+                                   "synthetic4", "synthetic8"])
 def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model):
     """The Hermitian-paired nb=2 kernels (Bloch vector when every k-point has
     the same trace, register pairs otherwise: real current; streamed pairs:
-    complex current of an asymmetric window) and the generic thread-per-k
-    kernel against the oracle on the same input."""
+    complex current of an asymmetric window, in Bloch-vector form when every
+    k-point has the same trace) and the generic thread-per-k kernel against
+    the oracle on the same input."""
     from hhgamd import tbgen
-    if model.startswith("synthetic"):
+    if model == "synthetic2_filled":
+        # on-site shift -0.5 eV: the lower band lies below E_F = 0 at every k-point
+        # (trace 1 everywhere): the streamed pairs in Bloch-vector form
+        lat, R, H, r = tbgen.synthetic(2, seed=77)
+        i0 = [tuple(x[:2]) for x in R].index((0, 0))
+        H[i0] += -0.5 * np.eye(2)
+        path = str(tmp_path / "syn2_filled_tb.dat")
+        tbgen.write_tb(path, lat, R, H, r)
+        expect = (11, 1)
+    elif model.startswith("synthetic"):
         nb = int(model[-1])
         path = str(tmp_path / f"syn{nb}_tb.dat")
         tbgen.write_tb(path, *tbgen.synthetic(nb, seed=77))
-        expect = {2: (4, 1), 4: (6, 3), 8: (6, 3)}[nb]  # streamed pairs / row-separable vs element-per-lane
+        # streamed pairs (the lower band empty at some k-points) / row-separable, vs element-per-lane
+        expect = {2: (4, 1), 4: (6, 3), 8: (6, 3)}[nb]
     elif model == "haldane_doped":
         # on-site shift -1 eV: near the K points both bands lie below E_F = 0, so the
         # occupation (trace of rho_k) is 2 there and 1 elsewhere: the 4-component pair kernel
@@ -156,7 +168,7 @@ def test_generic_and_hermitian_kernels_agree(native, oracle_lib, tmp_path, model
         with native.Solve(prm, device=False) as s:
             tr = np.trace(s.rho_k(), axis1=1, axis2=2).real
         assert tr.min() < 1.5 < tr.max()  # the occupation differs between k-points
-    if model == "synthetic2":
+    if model.startswith("synthetic2"):
         assert np.abs(Jref.imag).max() > 1e-6 * np.abs(Jref).max()  # the window current is complex here
 
 
