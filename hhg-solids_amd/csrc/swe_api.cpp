@@ -90,6 +90,17 @@ static void guarded(SWESim* s, F&& f) {
     }
 }
 
+// Appends one run's J(t) ([nt][6] doubles) to the run record. The record
+// grows by at least 32 runs at a time (virtual memory until written), so that
+// the first runs never copy the earlier ones when it grows.
+static void append_run(SWESim* s, const double* J, int nt) {
+    for (int a = 0; a < 3; ++a) {
+        std::vector<cplx>& v = s->J[a];
+        if (v.capacity() < v.size() + nt) v.reserve(std::max(2 * v.capacity(), v.size() + 32 * (size_t)nt));
+        for (int ti = 0; ti < nt; ++ti) v.push_back(cplx(J[(size_t)6 * ti + 2 * a], J[(size_t)6 * ti + 2 * a + 1]));
+    }
+}
+
 static hhg::SolveParams params_of(const Settings& st) {
     hhg::SolveParams p;
     p.nr1 = st.nr1; p.nr2 = st.nr2; p.nr3 = st.nr3;
@@ -149,8 +160,7 @@ API void SWESim_run_simulation(SWESim* s) {
             s->last_ms = hhg::dev_run(s->dev, s->precision, nullptr);
             J = hhg::dev_host_current(s->dev);
         }
-        for (int a = 0; a < 3; ++a)
-            for (int ti = 0; ti < nt; ++ti) s->J[a].push_back(cplx(J[(size_t)6 * ti + 2 * a], J[(size_t)6 * ti + 2 * a + 1]));
+        append_run(s, J, nt);
     });
 }
 
@@ -441,10 +451,7 @@ API int hhg_batch_run(hhg_batch* b) {
         b->last_ms = ms;
         for (size_t i = 0; i < b->sims.size(); ++i) {
             SWESim* s = b->sims[i];
-            const int nt = s->host->d.nt;
-            for (int a = 0; a < 3; ++a)
-                for (int ti = 0; ti < nt; ++ti)
-                    s->J[a].push_back(cplx(J[i][(size_t)6 * ti + 2 * a], J[i][(size_t)6 * ti + 2 * a + 1]));
+            append_run(s, J[i].data(), s->host->d.nt);
             s->last_ms = ms;
         }
     });
